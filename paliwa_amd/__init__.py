@@ -1,0 +1,27 @@
+"""paliwa_amd -- MI355X (gfx950) lifting-wavelet transform with the interface of
+freifrauvonbleifrei/paliwa's hierarchize / dehierarchize.
+
+The compute path is the HIP library paliwa_amd/lib/libpaliwa_hip.so (C ABI:
+include/paliwa_hip.h); this package is its Python host mirror.  Importing the
+package fails if the library has not been built.
+"""
+from .transform import (  # noqa: F401
+    FilterOffsetError,
+    HipError,
+    LevelError,
+    PaliwaError,
+    ShapeError,
+    UnknownWaveletError,
+    UnsupportedError,
+    dehierarchize,
+    dehierarchize_in,
+    hierarchize,
+    hierarchize_in,
+    lifting_wavelet_filter_offsets_and_coefficients,
+    lifting_wavelet_reconstruct_offsets_and_coefficients,
+    transform_in,
+    wavelet_filters,
+)
+from ._lib import LIBRARY_PATH  # noqa: F401
+
+__version__ = "0.1.0"

@@ -1,0 +1,122 @@
+"""ctypes binding of the C ABI declared in include/paliwa_hip.h.
+
+The library is built in-tree (paliwa_amd/lib/libpaliwa_hip.so, by
+``__graft_entry__.build()`` or ``make -C paliwa_amd/csrc``).  Importing this
+module loads it; a missing library raises immediately -- there is no fallback
+implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_HERE, "lib", "libpaliwa_hip.so")
+
+PALIWA_OK = 0
+PALIWA_ERR_INVALID_ARGUMENT = -1
+PALIWA_ERR_UNKNOWN_WAVELET = -2
+PALIWA_ERR_FILTER_OFFSET = -3
+PALIWA_ERR_LEVEL = -4
+PALIWA_ERR_SHAPE = -5
+PALIWA_ERR_DTYPE = -6
+PALIWA_ERR_UNSUPPORTED = -7
+PALIWA_ERR_HIP = -8
+
+PALIWA_FLOAT64 = 0
+PALIWA_FLOAT32 = 1
+
+PALIWA_MAX_DIMS = 26
+PALIWA_MAX_FILTERS = 16
+PALIWA_MAX_LEVELS = 64
+
+PATHS = {"auto": 0, "fused": 1, "lds_line": 2, "per_pass": 3}
+
+
+class paliwa_filter(ctypes.Structure):
+    _fields_ = [
+        ("offset", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("coefficients", ctypes.c_double * 3),
+    ]
+
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_fp = ctypes.POINTER(paliwa_filter)
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+
+# name -> (restype, argtypes); mirrors include/paliwa_hip.h
+SIGNATURES = {
+    "paliwa_last_error": (ctypes.c_char_p, []),
+    "paliwa_version": (ctypes.c_char_p, []),
+    "paliwa_wavelet_filters": (ctypes.c_int, [ctypes.c_char_p, _i32, _fp, _i32, _i32p]),
+    "paliwa_transform_in": (
+        ctypes.c_int,
+        [_vp, _i32, _i32, _i64p, _i32p, _i32p, _i32, _i32p, _i32, _fp, _i32, _vp],
+    ),
+    "paliwa_transform_in_path": (
+        ctypes.c_int,
+        [_vp, _i32, _i32, _i64p, _i32p, _i32p, _i32, _i32p, _i32, _fp, _i32, _i32, _vp],
+    ),
+    "paliwa_hierarchize_in": (
+        ctypes.c_int,
+        [_vp, _i32, _i32, _i64p, _i32p, _i32p, _i32p, _i32, ctypes.c_char_p, _vp],
+    ),
+    "paliwa_dehierarchize_in": (
+        ctypes.c_int,
+        [_vp, _i32, _i32, _i64p, _i32p, _i32p, _i32p, _i32, ctypes.c_char_p, _vp],
+    ),
+    "paliwa_hierarchize": (
+        ctypes.c_int,
+        [_vp, _i32, _i32, _i64p, _i32p, _i32p, _i32p, ctypes.c_char_p, _vp],
+    ),
+    "paliwa_dehierarchize": (
+        ctypes.c_int,
+        [_vp, _i32, _i32, _i64p, _i32p, _i32p, _i32p, ctypes.c_char_p, _vp],
+    ),
+}
+
+
+def _load() -> ctypes.CDLL:
+    if not os.path.exists(LIBRARY_PATH):
+        raise ImportError(
+            f"{LIBRARY_PATH} is missing: build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` or "
+            "`make -C paliwa_amd/csrc` (there is no non-HIP fallback)"
+        )
+    lib = ctypes.CDLL(LIBRARY_PATH)
+    for name, (restype, argtypes) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    return lib
+
+
+lib = _load()
+
+
+def i32_array(values):
+    values = [int(v) for v in values]
+    return (ctypes.c_int32 * max(1, len(values)))(*values)
+
+
+def i64_array(values):
+    values = [int(v) for v in values]
+    return (ctypes.c_int64 * max(1, len(values)))(*values)
+
+
+def filter_array(filters):
+    arr = (paliwa_filter * max(1, len(filters)))()
+    for i, (offset, coefficients) in enumerate(filters):
+        arr[i].offset = int(offset)
+        arr[i].reserved = 0
+        for k in range(3):
+            arr[i].coefficients[k] = float(coefficients[k])
+    return arr
+
+
+def last_error() -> str:
+    msg = lib.paliwa_last_error()
+    return msg.decode() if msg else ""

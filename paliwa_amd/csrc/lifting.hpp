@@ -1,0 +1,72 @@
+// Internal interface between the C ABI (capi.cpp) and the HIP kernels
+// (lifting_kernels.hip).  Not installed; the public interface is
+// include/paliwa_hip.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/paliwa_hip.h"
+
+namespace paliwa_hip {
+
+// One axis sweep: every line along the axis, every level of the level list in
+// order, every pass of the filter list in order.  The array is viewed as
+// [A][m][B]: A outer blocks, m stored points along the axis, B inner points.
+struct SweepParams {
+  void *data;
+  int64_t A;
+  int64_t B;
+  int32_t m;       // stored extent along the axis: n or n+1 (periodic image)
+  int32_t L;       // log2 of the lattice period n
+  int32_t nlevels; // entries of shift[]
+  int32_t npass;   // entries of offset[] / coef[]
+  int8_t shift[PALIWA_MAX_LEVELS];  // log2 storage stride of each level
+  int8_t offset[PALIWA_MAX_FILTERS]; // 0 even, 1 odd
+  double coef[PALIWA_MAX_FILTERS][3];
+  // Set by classify(): when the strides are 2^lo, 2^(lo+1), ..., 2^hi in
+  // increasing or decreasing order and the pass parities form one of the
+  // reference's table patterns, the fused kernel runs a compile-time ladder.
+  int32_t schedule; // kSchedule*
+  int32_t pattern;  // kPat* (ladder schedules only)
+  int32_t lo, hi;   // log2 stride range (ladder schedules only)
+};
+
+enum Schedule : int32_t {
+  kScheduleGeneric = 0,
+  kScheduleLadderAsc = 1,  // strides increase: hierarchize_in's level order
+  kScheduleLadderDesc = 2, // strides decrease: dehierarchize_in's level order
+};
+
+// Parity sequences of the passes in the reference's filter tables
+// (paliwa/paliwa_wavelets.hpp:15-28).
+enum Pattern : int32_t {
+  kPatOdd = 0,     // {odd}: hat, both directions
+  kPatOddEven = 1, // {odd, even}: biorthogonal forward, fullweighting inverse
+  kPatEvenOdd = 2, // {even, odd}: fullweighting forward, biorthogonal inverse
+  kPatNone = 3,
+};
+
+// Fills schedule/pattern/lo/hi from shift[], offset[].
+void classify(SweepParams &p);
+
+inline bool has_image(SweepParams const &p) {
+  return p.m == (int32_t(1) << p.L) + 1;
+}
+
+// Largest lattice period the fused (register) kernel handles per dtype.
+int fused_max_period(int dtype);
+// Largest lattice period the LDS-line kernel handles per dtype.
+int lds_line_max_period(int dtype);
+
+// Launchers; return a hipError_t.  The caller has validated p.
+hipError_t launch_fused(int dtype, SweepParams const &p, hipStream_t stream);
+hipError_t launch_fused_f64(SweepParams const &p, hipStream_t stream);
+hipError_t launch_fused_f32(SweepParams const &p, hipStream_t stream);
+hipError_t launch_lds_line(int dtype, SweepParams const &p,
+                           hipStream_t stream);
+hipError_t launch_per_pass(int dtype, SweepParams const &p,
+                           hipStream_t stream);
+
+} // namespace paliwa_hip

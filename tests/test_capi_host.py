@@ -1,0 +1,143 @@
+"""Host-side checks of the HIP library's C ABI (no GPU needed).
+
+The library must load, export every function include/paliwa_hip.h declares,
+return the reference's filter tables, and reject invalid arguments with the
+status codes that the C++ header maps onto the reference's exception types --
+all before any HIP call, so these run on a machine without a GPU.
+"""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+from paliwa_amd import _lib  # noqa: E402
+
+FAKE = ctypes.c_void_p(0x1000)  # never dereferenced: every call below fails validation
+
+
+def declared_functions():
+    names = []
+    for fname in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if fname.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fname)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names += re.findall(r"\b(paliwa_\w+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_function():
+    names = declared_functions()
+    assert "paliwa_hierarchize" in names and "paliwa_transform_in" in names
+    for name in names:
+        assert hasattr(_lib.lib, name), name
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+
+
+def test_version():
+    assert b"gfx950" in _lib.lib.paliwa_version()
+
+
+def test_filter_tables_match_reference():
+    from paliwa_amd import (lifting_wavelet_filter_offsets_and_coefficients as fwd,
+                            lifting_wavelet_reconstruct_offsets_and_coefficients as rec)
+
+    # paliwa/paliwa_wavelets.hpp:15-28
+    assert fwd["hat"] == [(1, (-0.5, 1.0, -0.5))]
+    assert fwd["biorthogonal"] == [(1, (-0.5, 1.0, -0.5)), (0, (0.25, 1.0, 0.25))]
+    assert fwd["fullweighting"] == [(0, (0.25, 0.5, 0.25)), (1, (-0.5, 1.0, -0.5))]
+    assert rec["hat"] == [(1, (0.5, 1.0, 0.5))]
+    assert rec["biorthogonal"] == [(0, (-0.25, 1.0, -0.25)), (1, (0.5, 1.0, 0.5))]
+    assert rec["fullweighting"] == [(1, (0.5, 1.0, 0.5)), (0, (-0.5, 2.0, -0.5))]
+
+
+def test_unknown_wavelet_name():
+    from paliwa_amd import UnknownWaveletError, wavelet_filters
+
+    with pytest.raises(UnknownWaveletError):
+        wavelet_filters("db4")
+    with pytest.raises(KeyError):  # std::out_of_range from std::map::at
+        wavelet_filters("haar")
+
+
+def call_named(fn, ndim=2, extents=(17, 16), level=(4, 4), lmin=(0, 0),
+               lmax=(4, 4), name=b"hat", dtype=_lib.PALIWA_FLOAT64, data=FAKE):
+    return fn(data, dtype, ndim, _lib.i64_array(extents), _lib.i32_array(level),
+              _lib.i32_array(lmin), _lib.i32_array(lmax), name, None)
+
+
+@pytest.mark.parametrize("fn_name", ["paliwa_hierarchize", "paliwa_dehierarchize"])
+def test_named_validation(fn_name):
+    fn = getattr(_lib.lib, fn_name)
+    assert call_named(fn, name=b"haar") == _lib.PALIWA_ERR_UNKNOWN_WAVELET
+    assert _lib.last_error() == "map::at"
+    assert call_named(fn, lmin=(5, 0)) == _lib.PALIWA_ERR_LEVEL
+    assert call_named(fn, lmin=(-1, 0)) == _lib.PALIWA_ERR_LEVEL
+    assert call_named(fn, lmax=(3, 4)) == _lib.PALIWA_ERR_LEVEL
+    assert call_named(fn, extents=(15, 16)) == _lib.PALIWA_ERR_SHAPE
+    assert call_named(fn, extents=(18, 16)) == _lib.PALIWA_ERR_SHAPE
+    assert call_named(fn, dtype=7) == _lib.PALIWA_ERR_DTYPE
+    assert call_named(fn, ndim=0) == _lib.PALIWA_ERR_UNSUPPORTED
+    assert call_named(fn, ndim=27) == _lib.PALIWA_ERR_UNSUPPORTED
+    assert call_named(fn, data=ctypes.c_void_p(0)) == _lib.PALIWA_ERR_INVALID_ARGUMENT
+
+
+def test_named_in_validation():
+    fn = _lib.lib.paliwa_hierarchize_in
+    args = lambda dim, name=b"hat": (
+        FAKE, _lib.PALIWA_FLOAT64, 2, _lib.i64_array((16, 16)),
+        _lib.i32_array((4, 4)), _lib.i32_array((0, 0)), _lib.i32_array((4, 4)),
+        dim, name, None)
+    assert fn(*args(2)) == _lib.PALIWA_ERR_INVALID_ARGUMENT
+    assert fn(*args(-1)) == _lib.PALIWA_ERR_INVALID_ARGUMENT
+    assert fn(*args(0, b"")) == _lib.PALIWA_ERR_UNKNOWN_WAVELET
+
+
+def call_transform_in(levels=(4, 3), filters=((1, (-0.5, 1.0, -0.5)),), dim=0,
+                      level=(4, 4), extents=(16, 16), path=0):
+    return _lib.lib.paliwa_transform_in_path(
+        FAKE, _lib.PALIWA_FLOAT64, 2, _lib.i64_array(extents),
+        _lib.i32_array(level), None, dim, _lib.i32_array(levels), len(levels),
+        _lib.filter_array(list(filters)), len(filters), path, None)
+
+
+def test_transform_in_validation():
+    # paliwa/paliwa_transform.hpp:239-240
+    assert call_transform_in(filters=((2, (0.0, 1.0, 0.0)),)) == _lib.PALIWA_ERR_FILTER_OFFSET
+    assert _lib.last_error() == "Filter offset not supported"
+    assert call_transform_in(filters=((-1, (0.0, 1.0, 0.0)),)) == _lib.PALIWA_ERR_FILTER_OFFSET
+    # paliwa/paliwa_transform.hpp:232 asserts l > 0
+    assert call_transform_in(levels=(0,)) == _lib.PALIWA_ERR_LEVEL
+    assert call_transform_in(levels=(5,)) == _lib.PALIWA_ERR_LEVEL
+    assert call_transform_in(dim=2) == _lib.PALIWA_ERR_INVALID_ARGUMENT
+    assert call_transform_in(path=9) == _lib.PALIWA_ERR_INVALID_ARGUMENT
+    many = tuple((1, (0.0, 1.0, 0.0)) for _ in range(17))
+    assert call_transform_in(filters=many) == _lib.PALIWA_ERR_UNSUPPORTED
+    assert call_transform_in(levels=tuple([1] * 65)) == _lib.PALIWA_ERR_UNSUPPORTED
+
+
+def test_python_exceptions_map_status_codes():
+    import paliwa_amd as p
+    from paliwa_amd.transform import check
+
+    for code, exc in (
+        (_lib.PALIWA_ERR_UNKNOWN_WAVELET, KeyError),
+        (_lib.PALIWA_ERR_FILTER_OFFSET, RuntimeError),
+        (_lib.PALIWA_ERR_LEVEL, ValueError),
+        (_lib.PALIWA_ERR_SHAPE, ValueError),
+        (_lib.PALIWA_ERR_HIP, RuntimeError),
+    ):
+        with pytest.raises(exc):
+            check(code)
+    with pytest.raises(p.FilterOffsetError):
+        check(_lib.PALIWA_ERR_FILTER_OFFSET)
+
+
+def test_python_api_rejects_host_tensors():
+    import torch
+    import paliwa_amd as p
+
+    with pytest.raises(p.ShapeError):
+        p.hierarchize(torch.zeros(16, 16, dtype=torch.float64), [4, 4], [0, 0], [4, 4])

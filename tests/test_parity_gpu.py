@@ -1,0 +1,216 @@
+"""Parity of the HIP library with the CPU oracle on the GPU.
+
+Float64 and float32 results must be bit-identical to the oracle's (the
+lifting update is evaluated in the reference's operand order with no fused
+multiply-add, so there is nothing to round differently).  At the full
+workload sizes the checks are the oracle (OpenMP schedule, bit-identical to
+the serial one) plus size-independent properties: round trip and mass.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle_bridge as ob
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+WAVELETS = ["hat", "biorthogonal", "fullweighting"]
+
+
+@pytest.fixture(scope="module")
+def pw():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a HIP device")
+    import paliwa_amd
+
+    return paliwa_amd
+
+
+def closed_random(rng, extents, level, dtype):
+    a = rng.uniform(-1.0, 1.0, size=extents).astype(dtype)
+    for d, l in enumerate(level):
+        if extents[d] == (1 << l) + 1:
+            src = [slice(None)] * len(extents)
+            dst = [slice(None)] * len(extents)
+            src[d], dst[d] = 0, 1 << l
+            a[tuple(dst)] = a[tuple(src)]
+    return np.ascontiguousarray(a)
+
+
+def gpu_named(pw, a, level, lmin, name, forward):
+    t = torch.from_numpy(a).cuda()
+    (pw.hierarchize if forward else pw.dehierarchize)(t, level, lmin, level, name)
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def oracle_named(a, level, lmin, name, forward, schedule=ob.SERIAL):
+    b = a.copy()
+    ob.transform(b, level, lmin, level, name, forward, schedule)
+    return b
+
+
+def assert_same(got, want, what):
+    if not np.array_equal(got, want):
+        diff = np.argwhere(got != want)
+        i = tuple(diff[0])
+        raise AssertionError(
+            f"{what}: {len(diff)} of {got.size} values differ; first at {i}: "
+            f"gpu {got[i]!r} oracle {want[i]!r}")
+
+
+def line_shapes(L):
+    n = 1 << L
+    return [
+        ([2, 1, L], [5, 3, n + 1]),   # contiguous axis, image, 15 lines
+        ([2, 1, L], [4, 2, n]),       # contiguous axis, no image
+        ([L, 2, 1], [n + 1, 5, 3]),   # strided axis, B = 15
+        ([L, 6], [n, 65]),            # strided axis, B = 65
+        ([L], [n + 1]),               # one line
+    ]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_every_line_length_bit_exact(pw, dtype):
+    rng = np.random.default_rng(11)
+    maxL = 11 if dtype == np.float64 else 12
+    for L in range(1, maxL + 1):
+        for level, extents in line_shapes(L):
+            for name in WAVELETS:
+                for lmin_L in sorted({0, L // 2, L - 1}):
+                    lmin = [min(lmin_L, l) for l in level]
+                    a = closed_random(rng, extents, level, dtype)
+                    for fwd in (True, False):
+                        got = gpu_named(pw, a, level, lmin, name, fwd)
+                        want = oracle_named(a, level, lmin, name, fwd)
+                        assert_same(got, want, f"L={L} {extents} {name} lmin={lmin} fwd={fwd}")
+
+
+@pytest.mark.parametrize("name", WAVELETS)
+def test_multidimensional_bit_exact(pw, name):
+    rng = np.random.default_rng(12)
+    cases = [
+        ([6, 5], [65, 33], [0, 2]),
+        ([4, 3, 5], [17, 9, 33], [0, 0, 1]),
+        ([3, 3, 2, 2, 1, 1], [9, 9, 5, 5, 3, 3], [0, 1, 0, 0, 0, 1]),
+        ([7, 2, 8], [128, 5, 257], [3, 0, 0]),
+    ]
+    for dtype in (np.float64, np.float32):
+        for level, extents, lmin in cases:
+            a = closed_random(rng, extents, level, dtype)
+            fwd = gpu_named(pw, a, level, lmin, name, True)
+            assert_same(fwd, oracle_named(a, level, lmin, name, True), f"{extents} fwd")
+            back = gpu_named(pw, fwd, level, lmin, name, False)
+            assert_same(back, oracle_named(fwd, level, lmin, name, False), f"{extents} inv")
+
+
+@pytest.mark.parametrize("path", ["fused", "lds_line", "per_pass"])
+def test_kernel_paths_bit_exact(pw, path):
+    rng = np.random.default_rng(13)
+    for dtype in (np.float64, np.float32):
+        for level, extents, dim in (([5, 3], [33, 9], 0), ([3, 7], [8, 129], 1),
+                                    ([6, 2, 4], [64, 5, 17], 1)):
+            for name in WAVELETS:
+                for table in (ob.FORWARD, ob.RECONSTRUCT):
+                    levels = list(range(level[dim], 0, -1))
+                    if table is ob.RECONSTRUCT:
+                        levels = levels[::-1]
+                    a = closed_random(rng, extents, level, dtype)
+                    t = torch.from_numpy(a).cuda()
+                    pw.transform_in(dim, t, level, level, levels, table[name], path=path)
+                    want = a.copy()
+                    ob.transform_in(want, level, dim, levels, table[name])
+                    assert_same(t.cpu().numpy(), want, f"{path} {extents} dim={dim} {name}")
+
+
+def test_generic_level_and_filter_lists(pw):
+    """Non-consecutive, repeated and partial level lists and 3..5-pass custom
+    filters (test/test_paliwa_plan.cpp:244-248 uses 5-pass filters)."""
+    rng = np.random.default_rng(14)
+    filter_sets = [
+        [(0, (0.5, 0.25, 0.125)), (1, (-0.5, 1.0, 0.125)), (1, (0.0, 1.0, 0.25))],
+        [(1, (-0.5, 1.0, 0.125)), (0, (0.0, 1.0, 0.25)), (1, (-0.5, 1.0, 0.125)),
+         (0, (0.0, 1.0, 0.25)), (1, (0.3, 0.7, -0.1))],
+        [(0, (0.25, 1.0, 0.25))],
+        [(1, (0.1, 1.0, 0.2)), (1, (-0.3, 1.0, 0.4))],
+    ]
+    level_lists = [[3, 1, 2], [5, 5, 2], [1, 2, 3, 4, 5], [2, 3], [4, 3], [5], [1]]
+    for dtype in (np.float64, np.float32):
+        for level, extents, dim in (([5, 4], [33, 16], 0), ([4, 5], [17, 33], 1),
+                                    ([3, 5, 2], [9, 32, 4], 1)):
+            for filters in filter_sets:
+                for levels in level_lists:
+                    levels = [l for l in levels if l <= level[dim]]
+                    if not levels:
+                        continue
+                    for path in ("auto", "lds_line", "per_pass"):
+                        a = closed_random(rng, extents, level, dtype)
+                        t = torch.from_numpy(a).cuda()
+                        pw.transform_in(dim, t, level, None, levels, filters, path=path)
+                        want = a.copy()
+                        ob.transform_in(want, level, dim, levels, filters)
+                        assert_same(t.cpu().numpy(), want,
+                                    f"{path} {extents} dim={dim} {levels} {len(filters)}p")
+
+
+def test_reference_multipass_kat_on_gpu(pw):
+    cases = json.load(open(os.path.join(GOLDEN, "plan_multipass.json")))
+    for c in cases:
+        a = np.array([float.fromhex(h) for h in c["input"]])
+        filters = [(o, tuple(f)) for o, f in c["filters"]]
+        for path in ("auto", "lds_line", "per_pass"):
+            t = torch.from_numpy(a).cuda()
+            pw.transform_in(0, t, [c["level"]], [c["maximum_level"]], c["levels"],
+                            filters, path=path)
+            expected = np.array([float.fromhex(h) for h in c["expected"]])
+            assert_same(t.cpu().numpy(), expected, f"{path} {c['passes']}:{c['mask']}")
+
+
+def test_golden_regression_vectors_on_gpu(pw):
+    manifest = json.load(open(os.path.join(GOLDEN, "regression.json")))
+    for m in manifest:
+        dt = np.dtype(m["dtype"])
+        load = lambda tag: np.fromfile(
+            os.path.join(GOLDEN, f"regression_{m['name']}_{tag}.raw"), dtype=dt
+        ).reshape(m["extents"])
+        a = load("input")
+        fwd = gpu_named(pw, a, m["level"], m["minimum_level"], m["wavelet"], True)
+        assert_same(fwd, load("hierarchized"), m["name"])
+        back = gpu_named(pw, fwd, m["level"], m["minimum_level"], m["wavelet"], False)
+        assert_same(back, load("dehierarchized"), m["name"])
+
+
+def test_long_lines_lds_and_per_pass(pw):
+    rng = np.random.default_rng(15)
+    for dtype, L, extents_fn in ((np.float64, 12, lambda n: [3, n + 1]),
+                                 (np.float64, 14, lambda n: [n]),
+                                 (np.float64, 15, lambda n: [n + 1]),
+                                 (np.float32, 13, lambda n: [n + 1, 3]),
+                                 (np.float32, 16, lambda n: [n])):
+        n = 1 << L
+        extents = extents_fn(n)
+        level = [L if e in (n, n + 1) else 1 for e in extents]
+        for name in ("hat", "biorthogonal"):
+            a = closed_random(rng, extents, level, dtype)
+            lmin = [0] * len(level)
+            got = gpu_named(pw, a, level, lmin, name, True)
+            assert_same(got, oracle_named(a, level, lmin, name, True), f"L={L} {extents}")
+
+
+def test_python_api_errors_on_gpu(pw):
+    t = torch.zeros(17, 16, dtype=torch.float64, device="cuda")
+    with pytest.raises(KeyError):
+        pw.hierarchize(t, [4, 4], [0, 0], [4, 4], "haar")
+    with pytest.raises(RuntimeError, match="Filter offset not supported"):
+        pw.transform_in(0, t, [4, 4], [4, 4], [4], [(3, (0.0, 1.0, 0.0))])
+    with pytest.raises(ValueError):
+        pw.hierarchize(t, [4, 4], [5, 0], [4, 4])
+    with pytest.raises(ValueError):
+        pw.hierarchize(torch.zeros(18, 16, dtype=torch.float64, device="cuda"),
+                       [4, 4], [0, 0], [4, 4])
+    with pytest.raises(ValueError):
+        pw.hierarchize(t.t(), [4, 4], [0, 0], [4, 4])  # not contiguous
