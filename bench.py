@@ -43,6 +43,8 @@ def parse():
                     help="skip the host-core baseline (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="wall-time budget of each CPU baseline schedule")
+    ap.add_argument("--no-ct", action="store_true",
+                    help="skip the combination-technique batch measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     return ap.parse_args()
@@ -93,6 +95,91 @@ def cpu_baseline(grid_cpu, budget_s):
         "sample": "257^3 biorthogonal float64 forward+inverse, C++/OpenMP "
                   "restatement of the reference's CPU schedules (oracle/), "
                   + "; ".join(detail),
+    }
+
+
+def ct_benchmark(torch, pw, fields, reps=5):
+    """Combination-technique set: 1325 4-D float64 grids, |l|_1 in 13..16,
+    l_i >= 1, hat basis, periodic images; cosine-product fields (seed 2).
+    Batched plan (one graph) vs one hierarchize call per grid (5300 kernel
+    launches), both timed with events on the same stream."""
+    import ctypes
+    import math
+
+    import numpy as np
+
+    from paliwa_amd import _lib
+
+    levels = pw.combination_levels(4, (13, 14, 15, 16), 1)
+    rng = np.random.default_rng(2)
+    lmin, lmax = [0] * 4, [13] * 4
+    grids = []
+    for l in levels:
+        a = rng.uniform(1.0, 4.0, size=4)
+        phi = rng.uniform(0.0, 2 * math.pi, size=4)
+        grids.append(fields.cosine_product(l, a, phi, device="cuda"))
+    initial = [g.clone() for g in grids]
+    points = sum(g.numel() for g in grids)
+    stream = torch.cuda.current_stream()
+    batch = pw.CombinationBatch(grids, levels, lmin, lmax)
+
+    def timed(fn, n):
+        fn()  # warm-up (graph capture, kernel attributes)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    batched_ms = timed(lambda: batch.hierarchize("hat"), reps)
+
+    # per grid: prebuilt C-ABI arguments, a tight loop of hierarchize calls
+    fn = _lib.lib.paliwa_hierarchize
+    h = ctypes.c_void_p(stream.cuda_stream)
+    mn, mx = _lib.i32_array(lmin), _lib.i32_array(lmax)
+    args = [(ctypes.c_void_p(g.data_ptr()), _lib.PALIWA_FLOAT64, 4,
+             _lib.i64_array(g.shape), _lib.i32_array(l), mn, mx, b"hat", h)
+            for g, l in zip(grids, levels)]
+
+    def per_grid():
+        for a in args:
+            fn(*a)
+
+    per_grid_ms = timed(per_grid, max(1, reps // 2))
+
+    # equality of the two forms on the same input
+    for g, i in zip(grids, initial):
+        g.copy_(i)
+    batch.hierarchize("hat")
+    out_batched = [g.clone() for g in grids]
+    for g, i in zip(grids, initial):
+        g.copy_(i)
+    per_grid()
+    torch.cuda.synchronize()
+    max_rel = 0.0
+    for a, b in zip(out_batched, grids):
+        d = (a - b).abs().max().item()
+        s = b.abs().max().item()
+        max_rel = max(max_rel, d / s if s > 0 else d)
+    info = batch.info()
+    batch.close()
+    bytes_fwd = 2 * 4 * points * 8
+    return {
+        "workload": "1325 4-D float64 grids, |l|_1 in {13..16}, l_i >= 1, hat, "
+                    "forward, periodic images",
+        "grids": len(levels),
+        "points": points,
+        "batched_ms": round(batched_ms, 4),
+        "per_grid_ms": round(per_grid_ms, 4),
+        "speedup": round(per_grid_ms / batched_ms, 2),
+        "batched_GBps": round(bytes_fwd / (batched_ms * 1e-3) / 1e9, 1),
+        "batched_launches": info["launches_per_direction"],
+        "per_grid_launches": 4 * len(levels),
+        "max_rel_diff_batched_vs_per_grid": max_rel,
     }
 
 
@@ -228,6 +315,8 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_ct:
+        record["ct"] = ct_benchmark(torch, pw, fields)
     if not args.no_cpu_baseline:
         record["cpu_baseline"] = cpu_baseline(original.cpu().numpy(), args.cpu_seconds)
     print(json.dumps(record), flush=True)

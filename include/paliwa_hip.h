@@ -144,6 +144,44 @@ int paliwa_dehierarchize(void *data, int32_t dtype, int32_t ndim,
                          const int32_t *maximum_level,
                          const char *wavelet_name, void *stream);
 
+/* ---- Batched transform of many grids (combination-technique sets) ------ */
+
+/* A plan over `ngrids` grids of one dtype and rank: grid g is data[g] with
+ * extents extents[g*ndim .. g*ndim+ndim) and level levels[g*ndim ..]; all
+ * grids share minimum_level and maximum_level.  The plan groups the axis
+ * sweeps of all grids by (axis, line period, stored extent) so that one
+ * launch covers every grid of a group, and replays a whole direction as one
+ * HIP graph.  Creating a plan validates every grid and allocates its
+ * descriptor tables on the current device (the only allocation; executing a
+ * plan allocates nothing).  The grids' memory must stay valid while the plan
+ * is used.  One plan must not execute concurrently on two streams.
+ * Replaces the reference's per-grid loop of paliwa::hierarchize calls over a
+ * combination-technique set (test/test_paliwa_combination_technique.cpp:
+ * 164-174); the result of each grid is bit-identical to hierarchize on it. */
+typedef struct paliwa_batch paliwa_batch;
+
+int paliwa_batch_create(paliwa_batch **plan, int32_t dtype, int32_t ndim,
+                        int32_t ngrids, void *const *data,
+                        const int64_t *extents, const int32_t *levels,
+                        const int32_t *minimum_level,
+                        const int32_t *maximum_level);
+
+/* hierarchize (forward) / dehierarchize (inverse) of every grid of the plan,
+ * asynchronous on `stream`. */
+int paliwa_batch_hierarchize(paliwa_batch *plan, const char *wavelet_name,
+                             void *stream);
+int paliwa_batch_dehierarchize(paliwa_batch *plan, const char *wavelet_name,
+                               void *stream);
+
+/* 0: launch the grouped kernels directly instead of replaying a graph. */
+int paliwa_batch_set_graphs(paliwa_batch *plan, int32_t enable);
+
+/* Kernel launches per direction, total stored points, descriptor bytes. */
+int paliwa_batch_info(const paliwa_batch *plan, int32_t *launches,
+                      int64_t *points, int64_t *descriptor_bytes);
+
+int paliwa_batch_destroy(paliwa_batch *plan);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -22,6 +22,7 @@ from .transform import (  # noqa: F401
     transform_in,
     wavelet_filters,
 )
+from .batch import CombinationBatch, combination_levels  # noqa: F401
 from ._lib import LIBRARY_PATH  # noqa: F401
 
 __version__ = "0.1.0"

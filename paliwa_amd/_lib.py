@@ -76,6 +76,16 @@ SIGNATURES = {
         ctypes.c_int,
         [_vp, _i32, _i32, _i64p, _i32p, _i32p, _i32p, ctypes.c_char_p, _vp],
     ),
+    "paliwa_batch_create": (
+        ctypes.c_int,
+        [ctypes.POINTER(_vp), _i32, _i32, _i32, ctypes.POINTER(_vp), _i64p, _i32p,
+         _i32p, _i32p],
+    ),
+    "paliwa_batch_hierarchize": (ctypes.c_int, [_vp, ctypes.c_char_p, _vp]),
+    "paliwa_batch_dehierarchize": (ctypes.c_int, [_vp, ctypes.c_char_p, _vp]),
+    "paliwa_batch_set_graphs": (ctypes.c_int, [_vp, _i32]),
+    "paliwa_batch_info": (ctypes.c_int, [_vp, _i32p, _i64p, _i64p]),
+    "paliwa_batch_destroy": (ctypes.c_int, [_vp]),
 }
 
 

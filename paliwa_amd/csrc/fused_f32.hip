@@ -25,4 +25,26 @@ hipError_t launch_fused_f32(SweepParams const &p, hipStream_t stream) {
   }
 }
 
+hipError_t launch_fused_batched_f32(SweepParams const &p, bool strided,
+                                    ManyGrids grids, int64_t tiles,
+                                    hipStream_t stream) {
+  using fused::launch_batched_kg;
+  using T = float;
+  switch (p.L) {
+  case 1: return launch_batched_kg<T, 2, 1>(p, strided, grids, tiles, stream);
+  case 2: return launch_batched_kg<T, 4, 1>(p, strided, grids, tiles, stream);
+  case 3: return launch_batched_kg<T, 8, 1>(p, strided, grids, tiles, stream);
+  case 4: return launch_batched_kg<T, 16, 1>(p, strided, grids, tiles, stream);
+  case 5: return launch_batched_kg<T, 32, 1>(p, strided, grids, tiles, stream);
+  case 6: return launch_batched_kg<T, 32, 2>(p, strided, grids, tiles, stream);
+  case 7: return launch_batched_kg<T, 32, 4>(p, strided, grids, tiles, stream);
+  case 8: return launch_batched_kg<T, 32, 8>(p, strided, grids, tiles, stream);
+  case 9: return launch_batched_kg<T, 32, 16>(p, strided, grids, tiles, stream);
+  case 10: return launch_batched_kg<T, 32, 32>(p, strided, grids, tiles, stream);
+  case 11: return launch_batched_kg<T, 32, 64>(p, strided, grids, tiles, stream);
+  case 12: return launch_batched_kg<T, 64, 64>(p, strided, grids, tiles, stream);
+  default: return hipErrorInvalidValue;
+  }
+}
+
 } // namespace paliwa_hip

@@ -16,11 +16,15 @@
 //
 // Staging.  Lines along the contiguous axis (B == 1) are copied between HBM
 // and LDS as one flat run (the tile is contiguous in memory); lines along a
-// strided axis are copied row by row, TB consecutive columns per row.  All the
-// loads of a thread are issued before the first LDS write so that K+1 loads
-// per lane are in flight.  Along a strided axis with one lane per line (G == 1,
-// short lines) each lane loads its own column straight into registers: the
-// TB lanes of a row read TB consecutive elements, which is already coalesced.
+// strided axis are copied row by row, TB consecutive columns per row.  Along a
+// strided axis with one lane per line (G == 1, short lines) each lane loads
+// its own column straight into registers: the TB lanes of a row read TB
+// consecutive elements, which is already coalesced.
+//
+// Pipelining.  The grid is persistent (as many workgroups as the device holds
+// at once); each workgroup walks its tiles and issues all HBM loads of the
+// next tile (K+1 per lane, into registers) before it computes and stores the
+// current one, so loads stay in flight while LDS, shuffles and stores work.
 //
 // Level schedules.  fused_ladder covers level lists whose storage strides are
 // consecutive powers of two in increasing (hierarchize) or decreasing
@@ -201,7 +205,6 @@ __device__ __forceinline__ void run_generic(T (&v)[K], int gbase, int gi,
 template <int K, int G> struct TileGeom {
   static constexpr int TB = kThreads / G;
   static constexpr int LOGK = ilog2(K);
-  static constexpr bool kDirect = false;
   __host__ __device__ static int line_pitch(int m) { return (m + G) | 1; }
   __host__ __device__ static int lds_elems(int m, bool strided) {
     if (strided && G == 1)
@@ -210,23 +213,167 @@ template <int K, int G> struct TileGeom {
   }
 };
 
-// Kernel body; MODE 0 = ladder (PAT, ASC used), 1 = generic.
-template <typename T, int K, int G, bool STRIDED, int MODE, int PAT, bool ASC>
-__device__ __forceinline__ void sweep_tile(SweepParams const &p,
-                                           uint32_t magic_m, T *lds) {
+// Per-thread staging state of one tile.
+template <typename T> struct Stage {
+  T *src = nullptr;  // contiguous axis: first point of the tile
+  T *colp = nullptr; // strided axis: this thread's column (nullptr: none)
+  int64_t B = 1;     // strided axis: element stride of the column
+  int count = 0;     // contiguous axis: points in the tile
+};
+
+// Issue every HBM load of one tile for this thread (K+1 per lane, all in
+// flight together).
+template <typename T, int K, int G, bool STRIDED>
+__device__ __forceinline__ Stage<T> load_tile(SweepParams const &p,
+                                              TileRef const &ref,
+                                              T (&buf)[K + 1]) {
   using Geom = TileGeom<K, G>;
   constexpr int TB = Geom::TB;
-  constexpr int LOGK = Geom::LOGK;
-  constexpr int ITER = K + 1; // staged points per thread (image included)
-
+  constexpr int ITER = K + 1;
   int const tid = threadIdx.x;
-  int64_t const tile = xcd_tile(blockIdx.x, gridDim.x);
+  int const m = p.m;
+  T *const g = static_cast<T *>(ref.data);
+  Stage<T> st;
+  if constexpr (!STRIDED) {
+    int64_t const line0 = ref.local * TB;
+    int64_t const left = ref.A - line0;
+    int const nl = static_cast<int>(left < TB ? left : TB);
+    st.src = g + line0 * m;
+    st.count = nl * m;
+#pragma unroll
+    for (int i = 0; i < ITER; ++i) {
+      int const e = tid + i * kThreads;
+      if (e < st.count)
+        buf[i] = st.src[e];
+    }
+  } else {
+    int const col = tid % TB;
+    int const row0 = tid / TB;
+    int64_t const c0 = ref.local * TB + col;
+    if (c0 < ref.A * ref.B) {
+      int64_t const a = c0 / ref.B;
+      int64_t const b = c0 - a * ref.B;
+      st.B = ref.B;
+      st.colp = g + a * m * ref.B + b;
+#pragma unroll
+      for (int i = 0; i < ITER; ++i) {
+        int const r = row0 + i * G;
+        if (r < m)
+          buf[i] = st.colp[r * st.B];
+      }
+    }
+  }
+  return st;
+}
+
+template <int K, int G, bool STRIDED>
+__device__ __forceinline__ int lds_index(int line, int r, int LP) {
+  using Geom = TileGeom<K, G>;
+  if constexpr (STRIDED)
+    return r * Geom::TB + line + (r >> Geom::LOGK);
+  else
+    return line * LP + r + (r >> Geom::LOGK);
+}
+
+// Registers holding a loaded tile -> LDS.
+template <typename T, int K, int G, bool STRIDED>
+__device__ __forceinline__ void stash_tile(SweepParams const &p,
+                                           Stage<T> const &st,
+                                           T const (&buf)[K + 1], T *lds,
+                                           uint32_t magic_m, int LP) {
+  using Geom = TileGeom<K, G>;
+  constexpr int ITER = K + 1;
+  int const tid = threadIdx.x;
+  int const m = p.m;
+  if constexpr (!STRIDED) {
+#pragma unroll
+    for (int i = 0; i < ITER; ++i) {
+      int const e = tid + i * kThreads;
+      if (e < st.count) {
+        int const l = __umulhi(static_cast<uint32_t>(e), magic_m);
+        int const r = e - l * m;
+        lds[lds_index<K, G, false>(l, r, LP)] = buf[i];
+      }
+    }
+  } else {
+    int const col = tid % Geom::TB;
+    int const row0 = tid / Geom::TB;
+    if (st.colp != nullptr) {
+#pragma unroll
+      for (int i = 0; i < ITER; ++i) {
+        int const r = row0 + i * G;
+        if (r < m)
+          lds[lds_index<K, G, true>(col, r, LP)] = buf[i];
+      }
+    }
+  }
+}
+
+// LDS -> HBM for one tile.
+template <typename T, int K, int G, bool STRIDED>
+__device__ __forceinline__ void store_tile(SweepParams const &p,
+                                           Stage<T> const &st, T const *lds,
+                                           uint32_t magic_m, int LP) {
+  using Geom = TileGeom<K, G>;
+  constexpr int ITER = K + 1;
+  int const tid = threadIdx.x;
+  int const m = p.m;
+  T out[ITER];
+  if constexpr (!STRIDED) {
+#pragma unroll
+    for (int i = 0; i < ITER; ++i) {
+      int const e = tid + i * kThreads;
+      if (e < st.count) {
+        int const l = __umulhi(static_cast<uint32_t>(e), magic_m);
+        int const r = e - l * m;
+        out[i] = lds[lds_index<K, G, false>(l, r, LP)];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < ITER; ++i) {
+      int const e = tid + i * kThreads;
+      if (e < st.count)
+        st.src[e] = out[i];
+    }
+  } else {
+    int const col = tid % Geom::TB;
+    int const row0 = tid / Geom::TB;
+    if (st.colp != nullptr) {
+#pragma unroll
+      for (int i = 0; i < ITER; ++i) {
+        int const r = row0 + i * G;
+        if (r < m)
+          out[i] = lds[lds_index<K, G, true>(col, r, LP)];
+      }
+#pragma unroll
+      for (int i = 0; i < ITER; ++i) {
+        int const r = row0 + i * G;
+        if (r < m)
+          st.colp[r * st.B] = out[i];
+      }
+    }
+  }
+}
+
+// Persistent kernel body.  Workgroup w processes tiles w', w'+W, w'+2W, ...
+// (W = grid size, w' = w regrouped by XCD so that consecutive tiles run on one
+// XCD).  The HBM loads of the next tile are issued before the current tile is
+// computed and stored, so every workgroup keeps loads in flight throughout.
+// MODE 0 = ladder (PAT, ASC used), 1 = generic level/filter lists.
+template <typename T, int K, int G, bool STRIDED, int MODE, int PAT, bool ASC,
+          typename Src>
+__device__ __forceinline__ void sweep_tiles(SweepParams const &p,
+                                            Src const &src, uint32_t magic_m,
+                                            int64_t ntiles, T *lds) {
+  using Geom = TileGeom<K, G>;
+  constexpr int TB = Geom::TB;
+  int const tid = threadIdx.x;
   int const m = p.m;
   int const n = 1 << p.L;
-  T *const g = static_cast<T *>(p.data);
   int const li = tid / G;
   int const gi = tid % G;
   int const gbase = (tid & 63) - gi;
+  uint32_t const nwg = gridDim.x;
 
   Coefs<T> c;
   if constexpr (MODE == 0) {
@@ -243,164 +390,135 @@ __device__ __forceinline__ void sweep_tile(SweepParams const &p,
       run_generic<T, K, G>(v, gbase, gi, p);
   };
 
-  T v[K];
+  int64_t t = xcd_tile(blockIdx.x, nwg);
   if constexpr (STRIDED && G == 1) {
-    // One lane per line: each lane's column straight from HBM.
-    int64_t const col = tile * TB + tid;
-    if (col >= p.A * p.B)
-      return;
-    int64_t const a = col / p.B;
-    int64_t const b = col - a * p.B;
-    T *const colp = g + a * m * p.B + b;
+    // One lane per line: each lane's column goes straight between HBM and
+    // registers; the TB lanes of a wavefront read TB consecutive columns.
+    struct Column {
+      T *p;
+      int64_t B;
+    };
+    auto column = [&](int64_t tile) -> Column {
+      if (tile >= ntiles)
+        return {nullptr, 1};
+      TileRef const ref = src(tile);
+      int64_t const col = ref.local * TB + tid;
+      if (col >= ref.A * ref.B)
+        return {nullptr, ref.B};
+      int64_t const a = col / ref.B;
+      return {static_cast<T *>(ref.data) + a * m * ref.B + (col - a * ref.B),
+              ref.B};
+    };
+    T buf[K];
+    Column cur = column(t);
+    if (cur.p != nullptr) {
 #pragma unroll
-    for (int j = 0; j < K; ++j)
-      v[j] = colp[j * p.B];
-    compute(v);
+      for (int j = 0; j < K; ++j)
+        buf[j] = cur.p[j * cur.B];
+    }
+    for (; t < ntiles; t += nwg) {
+      T v[K];
 #pragma unroll
-    for (int j = 0; j < K; ++j)
-      colp[j * p.B] = v[j];
-    if (m != n)
-      colp[static_cast<int64_t>(n) * p.B] = v[0];
-    return;
+      for (int j = 0; j < K; ++j)
+        v[j] = buf[j];
+      Column const next = column(t + nwg);
+      if (next.p != nullptr) {
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+          buf[j] = next.p[j * next.B];
+      }
+      if (cur.p != nullptr) {
+        compute(v);
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+          cur.p[j * cur.B] = v[j];
+        if (m != n)
+          cur.p[static_cast<int64_t>(n) * cur.B] = v[0];
+      }
+      cur = next;
+    }
   } else {
     int const LP = Geom::line_pitch(m);
-    auto lds_index = [&](int line, int r) {
-      if constexpr (STRIDED)
-        return r * TB + line + (r >> LOGK);
-      else
-        return line * LP + r + (r >> LOGK);
-    };
-
-    T buf[ITER];
-    T *src = nullptr;  // contiguous: first point of the tile
-    T *colp = nullptr; // strided: this thread's column
-    int count = 0;     // contiguous: points in the tile
-    int const col = tid % TB;
-    int const row0 = tid / TB;
-    if constexpr (!STRIDED) {
-      int64_t const line0 = tile * TB;
-      int64_t const left = p.A - line0;
-      int const nl = static_cast<int>(left < TB ? left : TB);
-      src = g + line0 * m;
-      count = nl * m;
+    T buf[K + 1];
+    Stage<T> cur;
+    if (t < ntiles)
+      cur = load_tile<T, K, G, STRIDED>(p, src(t), buf);
+    for (; t < ntiles; t += nwg) {
+      __syncthreads(); // the previous tile's LDS reads are done
+      stash_tile<T, K, G, STRIDED>(p, cur, buf, lds, magic_m, LP);
+      __syncthreads();
+      Stage<T> nxt;
+      if (t + nwg < ntiles)
+        nxt = load_tile<T, K, G, STRIDED>(p, src(t + nwg), buf);
+      T v[K];
 #pragma unroll
-      for (int i = 0; i < ITER; ++i) {
-        int const e = tid + i * kThreads;
-        if (e < count)
-          buf[i] = src[e];
-      }
+      for (int j = 0; j < K; ++j)
+        v[j] = lds[lds_index<K, G, STRIDED>(li, gi * K + j, LP)];
+      compute(v);
 #pragma unroll
-      for (int i = 0; i < ITER; ++i) {
-        int const e = tid + i * kThreads;
-        if (e < count) {
-          int const l = __umulhi(static_cast<uint32_t>(e), magic_m);
-          int const r = e - l * m;
-          lds[lds_index(l, r)] = buf[i];
-        }
-      }
-    } else {
-      int64_t const c0 = tile * TB + col;
-      if (c0 < p.A * p.B) {
-        int64_t const a = c0 / p.B;
-        int64_t const b = c0 - a * p.B;
-        colp = g + a * m * p.B + b;
-#pragma unroll
-        for (int i = 0; i < ITER; ++i) {
-          int const r = row0 + i * G;
-          if (r < m)
-            buf[i] = colp[r * p.B];
-        }
-#pragma unroll
-        for (int i = 0; i < ITER; ++i) {
-          int const r = row0 + i * G;
-          if (r < m)
-            lds[lds_index(col, r)] = buf[i];
-        }
-      }
-    }
-    __syncthreads();
-
-#pragma unroll
-    for (int j = 0; j < K; ++j)
-      v[j] = lds[lds_index(li, gi * K + j)];
-    compute(v);
-#pragma unroll
-    for (int j = 0; j < K; ++j)
-      lds[lds_index(li, gi * K + j)] = v[j];
-    if (m != n && gi == 0)
-      lds[lds_index(li, n)] = v[0];
-    __syncthreads();
-
-    if constexpr (!STRIDED) {
-#pragma unroll
-      for (int i = 0; i < ITER; ++i) {
-        int const e = tid + i * kThreads;
-        if (e < count) {
-          int const l = __umulhi(static_cast<uint32_t>(e), magic_m);
-          int const r = e - l * m;
-          buf[i] = lds[lds_index(l, r)];
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < ITER; ++i) {
-        int const e = tid + i * kThreads;
-        if (e < count)
-          src[e] = buf[i];
-      }
-    } else {
-      if (colp != nullptr) {
-#pragma unroll
-        for (int i = 0; i < ITER; ++i) {
-          int const r = row0 + i * G;
-          if (r < m)
-            buf[i] = lds[lds_index(col, r)];
-        }
-#pragma unroll
-        for (int i = 0; i < ITER; ++i) {
-          int const r = row0 + i * G;
-          if (r < m)
-            colp[r * p.B] = buf[i];
-        }
-      }
+      for (int j = 0; j < K; ++j)
+        lds[lds_index<K, G, STRIDED>(li, gi * K + j, LP)] = v[j];
+      if (m != n && gi == 0)
+        lds[lds_index<K, G, STRIDED>(li, n, LP)] = v[0];
+      __syncthreads();
+      store_tile<T, K, G, STRIDED>(p, cur, lds, magic_m, LP);
+      cur = nxt;
     }
   }
 }
 
 template <typename T, int K, int G, bool STRIDED, int PAT, bool ASC>
 __global__ __launch_bounds__(kThreads) void fused_ladder(SweepParams p,
-                                                         uint32_t magic_m) {
+                                                         uint32_t magic_m,
+                                                         int64_t ntiles) {
   extern __shared__ unsigned char smem_raw[];
-  sweep_tile<T, K, G, STRIDED, 0, PAT, ASC>(p, magic_m,
-                                            reinterpret_cast<T *>(smem_raw));
+  sweep_tiles<T, K, G, STRIDED, 0, PAT, ASC>(p, OneGrid{p.data, p.A, p.B},
+                                             magic_m, ntiles,
+                                             reinterpret_cast<T *>(smem_raw));
+}
+
+// The same sweep over the tiles of many grids with equal line length along
+// the axis (one launch of a batched transform).
+template <typename T, int K, int G, bool STRIDED, int PAT, bool ASC>
+__global__ __launch_bounds__(kThreads) void fused_ladder_batched(
+    SweepParams p, uint32_t magic_m, int64_t ntiles, ManyGrids grids) {
+  extern __shared__ unsigned char smem_raw[];
+  sweep_tiles<T, K, G, STRIDED, 0, PAT, ASC>(p, grids, magic_m, ntiles,
+                                             reinterpret_cast<T *>(smem_raw));
 }
 
 template <typename T, int K, int G, bool STRIDED>
 __global__ __launch_bounds__(kThreads) void fused_generic(SweepParams p,
-                                                          uint32_t magic_m) {
+                                                          uint32_t magic_m,
+                                                          int64_t ntiles) {
   extern __shared__ unsigned char smem_raw[];
-  sweep_tile<T, K, G, STRIDED, 1, 0, true>(p, magic_m,
-                                           reinterpret_cast<T *>(smem_raw));
+  sweep_tiles<T, K, G, STRIDED, 1, 0, true>(p, OneGrid{p.data, p.A, p.B},
+                                            magic_m, ntiles,
+                                            reinterpret_cast<T *>(smem_raw));
 }
 
-template <typename Kernel>
-hipError_t allow_lds(Kernel kernel, size_t bytes) {
-  if (bytes <= 65536)
-    return hipSuccess;
-  return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             static_cast<int>(bytes));
-}
+// Number of workgroups of `kernel` the device holds at once (all CUs), cached
+// per (kernel, LDS size): the persistent grid.
+int resident_workgroups(const void *kernel, size_t lds_bytes);
 
-template <typename Kernel>
+template <typename Kernel, typename... Extra>
 hipError_t launch_one(Kernel kernel, SweepParams const &p, size_t bytes,
-                      uint32_t tiles, hipStream_t stream) {
-  hipError_t err = allow_lds(kernel, bytes);
-  if (err != hipSuccess)
-    return err;
+                      int64_t tiles, hipStream_t stream, Extra... extra) {
+  if (bytes > 65536) {
+    hipError_t err = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(kernel),
+        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+    if (err != hipSuccess)
+      return err;
+  }
+  int64_t const resident =
+      resident_workgroups(reinterpret_cast<const void *>(kernel), bytes);
+  uint32_t const grid =
+      static_cast<uint32_t>(tiles < resident ? tiles : resident);
   // e / m by multiply-high: exact while (points per tile) * m < 2^32.
   uint32_t const magic = static_cast<uint32_t>(0xffffffffu / p.m + 1);
-  hipLaunchKernelGGL(kernel, dim3(tiles), dim3(kThreads), bytes, stream, p,
-                     magic);
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), bytes, stream, p,
+                     magic, tiles, extra...);
   return hipGetLastError();
 }
 
@@ -409,33 +527,63 @@ hipError_t launch_kg_layout(SweepParams const &p, hipStream_t stream) {
   using Geom = TileGeom<K, G>;
   int64_t const lines = p.A * p.B;
   int64_t const tiles = (lines + Geom::TB - 1) / Geom::TB;
-  if (tiles > 0x7fffffffLL)
-    return hipErrorInvalidValue;
   size_t const bytes = sizeof(T) * Geom::lds_elems(p.m, STRIDED);
-  uint32_t const nt = static_cast<uint32_t>(tiles);
   if (p.schedule == kScheduleLadderAsc || p.schedule == kScheduleLadderDesc) {
     bool const asc = p.schedule == kScheduleLadderAsc;
     switch (p.pattern) {
     case kPatOdd:
-      return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatOdd, true>, p, bytes, nt, stream)
-                 : launch_one(fused_ladder<T, K, G, STRIDED, kPatOdd, false>, p, bytes, nt, stream);
+      return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatOdd, true>, p, bytes, tiles, stream)
+                 : launch_one(fused_ladder<T, K, G, STRIDED, kPatOdd, false>, p, bytes, tiles, stream);
     case kPatOddEven:
-      return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatOddEven, true>, p, bytes, nt, stream)
-                 : launch_one(fused_ladder<T, K, G, STRIDED, kPatOddEven, false>, p, bytes, nt, stream);
+      return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatOddEven, true>, p, bytes, tiles, stream)
+                 : launch_one(fused_ladder<T, K, G, STRIDED, kPatOddEven, false>, p, bytes, tiles, stream);
     case kPatEvenOdd:
-      return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatEvenOdd, true>, p, bytes, nt, stream)
-                 : launch_one(fused_ladder<T, K, G, STRIDED, kPatEvenOdd, false>, p, bytes, nt, stream);
+      return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatEvenOdd, true>, p, bytes, tiles, stream)
+                 : launch_one(fused_ladder<T, K, G, STRIDED, kPatEvenOdd, false>, p, bytes, tiles, stream);
     default:
       break;
     }
   }
-  return launch_one(fused_generic<T, K, G, STRIDED>, p, bytes, nt, stream);
+  return launch_one(fused_generic<T, K, G, STRIDED>, p, bytes, tiles, stream);
 }
 
 template <typename T, int K, int G>
 hipError_t launch_kg(SweepParams const &p, hipStream_t stream) {
   return p.B > 1 ? launch_kg_layout<T, K, G, true>(p, stream)
                  : launch_kg_layout<T, K, G, false>(p, stream);
+}
+
+// Batched launches exist for the ladder schedules only (the named
+// hierarchize / dehierarchize transforms).
+template <typename T, int K, int G, bool STRIDED>
+hipError_t launch_batched_layout(SweepParams const &p, ManyGrids grids,
+                                 int64_t tiles, hipStream_t stream) {
+  using Geom = TileGeom<K, G>;
+  size_t const bytes = sizeof(T) * Geom::lds_elems(p.m, STRIDED);
+  bool const asc = p.schedule == kScheduleLadderAsc;
+  if (p.schedule != kScheduleLadderAsc && p.schedule != kScheduleLadderDesc)
+    return hipErrorInvalidValue;
+  switch (p.pattern) {
+  case kPatOdd:
+    return asc ? launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOdd, true>, p, bytes, tiles, stream, grids)
+               : launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOdd, false>, p, bytes, tiles, stream, grids);
+  case kPatOddEven:
+    return asc ? launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOddEven, true>, p, bytes, tiles, stream, grids)
+               : launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOddEven, false>, p, bytes, tiles, stream, grids);
+  case kPatEvenOdd:
+    return asc ? launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatEvenOdd, true>, p, bytes, tiles, stream, grids)
+               : launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatEvenOdd, false>, p, bytes, tiles, stream, grids);
+  default:
+    return hipErrorInvalidValue;
+  }
+}
+
+template <typename T, int K, int G>
+hipError_t launch_batched_kg(SweepParams const &p, bool strided,
+                             ManyGrids grids, int64_t tiles,
+                             hipStream_t stream) {
+  return strided ? launch_batched_layout<T, K, G, true>(p, grids, tiles, stream)
+                 : launch_batched_layout<T, K, G, false>(p, grids, tiles, stream);
 }
 
 } // namespace fused

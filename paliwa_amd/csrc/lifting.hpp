@@ -48,6 +48,39 @@ enum Pattern : int32_t {
   kPatNone = 3,
 };
 
+// Where a tile lives: its grid viewed as [A][m][B] and the tile's index
+// within that grid.
+struct TileRef {
+  void *data;
+  int64_t A, B;
+  int64_t local;
+};
+
+// Tiles of one grid (the single-grid entry points).
+struct OneGrid {
+  void *data;
+  int64_t A, B;
+  __device__ TileRef operator()(int64_t t) const { return {data, A, B, t}; }
+};
+
+// Tiles of many grids (the batched entry points): tile t belongs to grid
+// tile_grid[t], whose descriptor gives its storage, its [A][m][B] view and
+// the index of its first tile.
+struct BatchGrid {
+  void *data;
+  int64_t A, B;
+  int64_t tile_base;
+};
+
+struct ManyGrids {
+  BatchGrid const *grids;
+  int32_t const *tile_grid;
+  __device__ TileRef operator()(int64_t t) const {
+    BatchGrid const d = grids[tile_grid[t]];
+    return {d.data, d.A, d.B, t - d.tile_base};
+  }
+};
+
 // Fills schedule/pattern/lo/hi from shift[], offset[].
 void classify(SweepParams &p);
 
@@ -64,6 +97,33 @@ int lds_line_max_period(int dtype);
 hipError_t launch_fused(int dtype, SweepParams const &p, hipStream_t stream);
 hipError_t launch_fused_f64(SweepParams const &p, hipStream_t stream);
 hipError_t launch_fused_f32(SweepParams const &p, hipStream_t stream);
+
+// Points per lane (chunk length K) of the fused kernel for a line of period
+// 2^L; the kernel's tile holds 256*K / 2^L lines.
+inline int fused_chunk(int dtype, int L) {
+  int const target = dtype == PALIWA_FLOAT64 ? 16 : 32; // 128 bytes per lane
+  int const n = 1 << L;
+  if (n <= target)
+    return n;
+  if (n / target <= 64)
+    return target;
+  return n / 64;
+}
+inline int fused_tile_lines(int dtype, int L) {
+  return 256 * fused_chunk(dtype, L) / (1 << L);
+}
+
+// Batched fused sweep: `tiles` tiles of the grids in `grids`, all with line
+// period 2^p.L and stored extent p.m along the swept axis; p.data/A/B unused.
+hipError_t launch_fused_batched(int dtype, SweepParams const &p, bool strided,
+                                ManyGrids grids, int64_t tiles,
+                                hipStream_t stream);
+hipError_t launch_fused_batched_f64(SweepParams const &p, bool strided,
+                                    ManyGrids grids, int64_t tiles,
+                                    hipStream_t stream);
+hipError_t launch_fused_batched_f32(SweepParams const &p, bool strided,
+                                    ManyGrids grids, int64_t tiles,
+                                    hipStream_t stream);
 hipError_t launch_lds_line(int dtype, SweepParams const &p,
                            hipStream_t stream);
 hipError_t launch_per_pass(int dtype, SweepParams const &p,

@@ -17,9 +17,13 @@
 //                   for lines longer than the fused kernel holds in registers.
 //   per_pass_sweep  one launch per (level, pass) directly on HBM, the launch
 //                   structure of the reference; for any length.
+#include "fused_sweep.hpp"
 #include "lifting.hpp"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
 
 namespace paliwa_hip {
 namespace {
@@ -199,6 +203,38 @@ void classify(SweepParams &p) {
   }
 }
 
+namespace fused {
+
+int resident_workgroups(const void *kernel, size_t lds_bytes) {
+  static std::mutex mutex;
+  static std::map<std::pair<int, std::pair<const void *, size_t>>, int> cache;
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess)
+    return 1024;
+  auto const key = std::make_pair(device, std::make_pair(kernel, lds_bytes));
+  {
+    std::lock_guard<std::mutex> lock(mutex);
+    auto it = cache.find(key);
+    if (it != cache.end())
+      return it->second;
+  }
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kThreads,
+                                                   lds_bytes) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                            device) != hipSuccess ||
+      cus < 1)
+    cus = 256;
+  int const resident = per_cu * cus;
+  std::lock_guard<std::mutex> lock(mutex);
+  cache[key] = resident;
+  return resident;
+}
+
+} // namespace fused
+
 int fused_max_period(int dtype) {
   return dtype == PALIWA_FLOAT64 ? 2048 : 4096;
 }
@@ -210,6 +246,14 @@ int lds_line_max_period(int dtype) {
 hipError_t launch_fused(int dtype, SweepParams const &p, hipStream_t stream) {
   return dtype == PALIWA_FLOAT64 ? launch_fused_f64(p, stream)
                                  : launch_fused_f32(p, stream);
+}
+
+hipError_t launch_fused_batched(int dtype, SweepParams const &p, bool strided,
+                                ManyGrids grids, int64_t tiles,
+                                hipStream_t stream) {
+  return dtype == PALIWA_FLOAT64
+             ? launch_fused_batched_f64(p, strided, grids, tiles, stream)
+             : launch_fused_batched_f32(p, strided, grids, tiles, stream);
 }
 
 hipError_t launch_lds_line(int dtype, SweepParams const &p,

@@ -1,0 +1,138 @@
+// Exercises the C++ drop-in header include/paliwa/paliwa_transform.hpp the way
+// the reference's own tests call paliwa (test/test_paliwa_transform.cpp):
+// hierarchize / dehierarchize / *_in / transform_in on device grids, checked
+// bit-for-bit against the CPU oracle (test infrastructure), plus the exception
+// types of the reference.
+//   test_dropin --host-only   exception checks that fail validation before any
+//                             HIP call (runs without a GPU)
+//   test_dropin               everything (needs a GPU)
+#include <paliwa/paliwa_transform.hpp>
+
+#include "../../oracle/paliwa_oracle.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#define CHECK(cond)                                                            \
+  do {                                                                         \
+    if (!(cond)) {                                                             \
+      std::fprintf(stderr, "FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);   \
+      std::exit(1);                                                            \
+    }                                                                          \
+  } while (0)
+
+#define HIP_CHECK(x) CHECK((x) == hipSuccess)
+
+template <typename E, typename F> bool throws(F &&f) {
+  try {
+    f();
+  } catch (E const &) {
+    return true;
+  } catch (...) {
+    return false;
+  }
+  return false;
+}
+
+static void host_only_checks() {
+  // Never dereferenced: every call below fails argument validation.
+  paliwa::GridSpan<double, 2> g{reinterpret_cast<double *>(0x1000), {17, 16}};
+  paliwa::LevelVector<2> lv{4, 4}, mn{0, 0}, mx{4, 4};
+  CHECK(throws<std::out_of_range>([&] { paliwa::hierarchize(g, lv, mn, mx, "haar"); }));
+  CHECK(throws<std::out_of_range>([&] { paliwa::dehierarchize_in<1>(g, lv, mn, mx, "db2"); }));
+  std::vector<std::pair<int, std::array<double, 3>>> bad{{2, {0.0, 1.0, 0.0}}};
+  std::vector<long> levels{4, 3};
+  bool got = false;
+  try {
+    paliwa::transform_in<0>(g, lv, mx, levels, bad);
+  } catch (std::runtime_error const &e) {
+    got = std::strcmp(e.what(), "Filter offset not supported") == 0;
+  }
+  CHECK(got);
+  paliwa::LevelVector<2> big{5, 0};
+  CHECK(throws<std::invalid_argument>([&] { paliwa::hierarchize(g, lv, big, mx, "hat"); }));
+  paliwa::GridSpan<double, 2> wrong{reinterpret_cast<double *>(0x1000), {18, 16}};
+  CHECK(throws<std::invalid_argument>([&] { paliwa::hierarchize(wrong, lv, mn, mx); }));
+  CHECK(paliwa::lifting_wavelet_filter_offsets_and_coefficients.at("biorthogonal").size() == 2);
+  std::printf("host-only checks OK\n");
+}
+
+template <typename T>
+static void grid_checks(hipStream_t stream) {
+  // 3-D grid with images on two axes, the reference's level/minimum-level
+  // arguments, every wavelet, forward then inverse.
+  constexpr std::size_t D = 3;
+  paliwa::LevelVector<D> lv{5, 3, 6}, mn{1, 0, 2}, mx{6, 3, 7};
+  std::array<int64_t, D> ext{33, 8, 65};
+  std::size_t const n = ext[0] * ext[1] * ext[2];
+  std::mt19937 rng(20260928);
+  std::uniform_real_distribution<double> u(-1.0, 1.0);
+  std::vector<T> host(n);
+  for (auto &x : host)
+    x = static_cast<T>(u(rng));
+  // close the images (index 2^l is a copy of index 0)
+  for (int64_t i = 0; i < ext[0]; ++i)
+    for (int64_t j = 0; j < ext[1]; ++j)
+      host[(i * ext[1] + j) * ext[2] + 64] = host[(i * ext[1] + j) * ext[2]];
+  for (int64_t j = 0; j < ext[1]; ++j)
+    for (int64_t k = 0; k < ext[2]; ++k)
+      host[(32 * ext[1] + j) * ext[2] + k] = host[j * ext[2] + k];
+  T *dev = nullptr;
+  HIP_CHECK(hipMalloc(&dev, n * sizeof(T)));
+  paliwa::GridSpan<T, D> grid{dev, ext};
+  std::vector<int> level(lv.begin(), lv.end()), lmin(mn.begin(), mn.end()),
+      lmax(mx.begin(), mx.end());
+  std::vector<int64_t> extents(ext.begin(), ext.end());
+  for (char const *name : {"hat", "biorthogonal", "fullweighting"}) {
+    HIP_CHECK(hipMemcpyAsync(dev, host.data(), n * sizeof(T), hipMemcpyHostToDevice, stream));
+    paliwa::hierarchize(grid, lv, mn, mx, name, paliwa::HipExec{stream});
+    std::vector<T> got(n), want = host;
+    HIP_CHECK(hipMemcpyAsync(got.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    paliwa_oracle::hierarchize(want.data(), extents, level, lmin, lmax, name);
+    CHECK(std::memcmp(got.data(), want.data(), n * sizeof(T)) == 0);
+    // inverse axis by axis through the *_in entry points
+    paliwa::dehierarchize_in<0>(grid, lv, mn, mx, name, paliwa::HipExec{stream});
+    paliwa::dehierarchize_in<1>(grid, lv, mn, mx, name, paliwa::HipExec{stream});
+    paliwa::dehierarchize_in<2>(grid, lv, mn, mx, name, paliwa::HipExec{stream});
+    HIP_CHECK(hipMemcpyAsync(got.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    paliwa_oracle::dehierarchize(want.data(), extents, level, lmin, lmax, name);
+    CHECK(std::memcmp(got.data(), want.data(), n * sizeof(T)) == 0);
+    double err = 0.0;
+    for (std::size_t i = 0; i < n; ++i)
+      err = std::fmax(err, std::fabs(double(got[i]) - double(host[i])));
+    CHECK(err < (sizeof(T) == 8 ? 1e-14 : 3e-6));
+  }
+  // transform_in with a custom 3-pass filter list and a non-consecutive
+  // level list (the generic schedule)
+  std::vector<std::pair<int, std::array<double, 3>>> filters{
+      {1, {-0.5, 1.0, 0.125}}, {0, {0.0, 1.0, 0.25}}, {1, {0.3, 0.7, -0.1}}};
+  std::vector<long> levels{6, 2, 4};
+  HIP_CHECK(hipMemcpyAsync(dev, host.data(), n * sizeof(T), hipMemcpyHostToDevice, stream));
+  paliwa::transform_in<2>(grid, lv, mx, levels, filters, paliwa::HipExec{stream});
+  std::vector<T> got(n), want = host;
+  HIP_CHECK(hipMemcpyAsync(got.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+  HIP_CHECK(hipStreamSynchronize(stream));
+  paliwa_oracle::transform_in(want.data(), extents, level, 2, {6, 2, 4}, filters);
+  CHECK(std::memcmp(got.data(), want.data(), n * sizeof(T)) == 0);
+  HIP_CHECK(hipFree(dev));
+}
+
+int main(int argc, char **argv) {
+  host_only_checks();
+  if (argc > 1 && std::strcmp(argv[1], "--host-only") == 0)
+    return 0;
+  hipStream_t stream;
+  HIP_CHECK(hipStreamCreate(&stream));
+  grid_checks<double>(stream);
+  grid_checks<float>(stream);
+  HIP_CHECK(hipStreamDestroy(stream));
+  std::printf("drop-in header checks OK\n");
+  return 0;
+}

@@ -1,0 +1,123 @@
+"""Batched (combination-technique) transform on the GPU: every grid's result
+must be bit-identical to the single-grid transform and to the CPU oracle."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle_bridge as ob
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pw():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a HIP device")
+    import paliwa_amd
+
+    return paliwa_amd
+
+
+def make_grids(levels, image, dtype, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for l in levels:
+        ext = [(1 << x) + (1 if image else 0) for x in l]
+        a = rng.uniform(-1, 1, size=ext).astype(dtype)
+        for d, x in enumerate(l):
+            if image:
+                src = [slice(None)] * len(l)
+                dst = [slice(None)] * len(l)
+                src[d], dst[d] = 0, 1 << x
+                a[tuple(dst)] = a[tuple(src)]
+        out.append(np.ascontiguousarray(a))
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("image", [True, False])
+def test_batch_matches_oracle(pw, dtype, image):
+    levels = pw.combination_levels(3, (6, 7), 1)
+    lmin, lmax = [1, 0, 1], [5, 5, 5]
+    arrays = make_grids(levels, image, dtype, seed=21)
+    for name in ("hat", "biorthogonal", "fullweighting"):
+        for graphs in (True, False):
+            grids = [torch.from_numpy(a).cuda() for a in arrays]
+            batch = pw.CombinationBatch(grids, levels, lmin, lmax)
+            batch.use_graphs(graphs)
+            batch.hierarchize(name)
+            torch.cuda.synchronize()
+            for a, g, l in zip(arrays, grids, levels):
+                want = a.copy()
+                ob.hierarchize(want, list(l), lmin, lmax, name)
+                assert np.array_equal(g.cpu().numpy(), want), (name, l, graphs)
+            batch.dehierarchize(name)
+            torch.cuda.synchronize()
+            for a, g, l in zip(arrays, grids, levels):
+                want = a.copy()
+                ob.hierarchize(want, list(l), lmin, lmax, name)
+                ob.dehierarchize(want, list(l), lmin, lmax, name)
+                assert np.array_equal(g.cpu().numpy(), want), (name, l, graphs)
+            batch.close()
+
+
+def test_batch_replays_graph_repeatedly(pw):
+    """Graph replays interleaved with eager work: every replay must equal the
+    eager result."""
+    levels = pw.combination_levels(2, (7, 8), 1)
+    arrays = make_grids(levels, True, np.float64, seed=22)
+    ref = [torch.from_numpy(a).cuda() for a in arrays]
+    for g, l in zip(ref, levels):
+        pw.hierarchize(g, l, [0, 0], [7, 7], "biorthogonal")
+    grids = [torch.from_numpy(a).cuda() for a in arrays]
+    batch = pw.CombinationBatch(grids, levels, [0, 0], [7, 7])
+    for _ in range(3):
+        for g, a in zip(grids, arrays):
+            g.copy_(torch.from_numpy(a))
+        torch.cuda.synchronize()
+        batch.hierarchize("biorthogonal")
+        torch.cuda.synchronize()
+        for g, r in zip(grids, ref):
+            assert torch.equal(g, r)
+
+
+def test_combination_technique_set_1325_grids(pw):
+    """The 4-D set |l|_1 in {13..16}, l_i >= 1: 1325 grids, hat basis,
+    float64, periodic images, batched vs one hierarchize per grid."""
+    from paliwa_amd import fields
+
+    levels = pw.combination_levels(4, (13, 14, 15, 16), 1)
+    assert len(levels) == 1325
+    rng = np.random.default_rng(2)
+    lmin, lmax = [0] * 4, [13] * 4
+    grids, singles = [], []
+    for l in levels:
+        a = rng.uniform(1.0, 4.0, size=4)
+        phi = rng.uniform(0.0, 2 * math.pi, size=4)
+        g = fields.cosine_product(l, a, phi, device="cuda")
+        grids.append(g)
+        singles.append(g.clone())
+    points = sum(g.numel() for g in grids)
+    assert points == 90963469
+    sample = list(range(0, 1325, 97))
+    initial = {i: grids[i].cpu().numpy() for i in sample}
+    batch = pw.CombinationBatch(grids, levels, lmin, lmax)
+    batch.hierarchize("hat")
+    for g, l in zip(singles, levels):
+        pw.hierarchize(g, l, lmin, lmax, "hat")
+    torch.cuda.synchronize()
+    for g, s in zip(grids, singles):
+        assert torch.equal(g, s)
+    # a sample against the CPU oracle
+    for i in sample:
+        want = initial[i].copy()
+        ob.hierarchize(want, list(levels[i]), lmin, lmax, "hat")
+        assert np.array_equal(grids[i].cpu().numpy(), want), levels[i]
+    batch.dehierarchize("hat")
+    torch.cuda.synchronize()
+    for i in sample:
+        err = np.abs(grids[i].cpu().numpy() - initial[i]).max()
+        assert err <= 1e-14, (levels[i], err)
+    batch.close()

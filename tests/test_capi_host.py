@@ -141,3 +141,32 @@ def test_python_api_rejects_host_tensors():
 
     with pytest.raises(p.ShapeError):
         p.hierarchize(torch.zeros(16, 16, dtype=torch.float64), [4, 4], [0, 0], [4, 4])
+
+
+def test_batch_create_validation():
+    plan = ctypes.c_void_p(None)
+    ptrs = (ctypes.c_void_p * 2)(0x1000, 0x2000)
+    ext = _lib.i64_array([17, 16, 9, 8])
+    lv = _lib.i32_array([4, 4, 3, 3])
+    mn, mx = _lib.i32_array([0, 0]), _lib.i32_array([4, 4])
+    create = _lib.lib.paliwa_batch_create
+    bad_ext = _lib.i64_array([17, 16, 10, 8])
+    assert create(ctypes.byref(plan), 0, 2, 2, ptrs, bad_ext, lv, mn, mx) == _lib.PALIWA_ERR_SHAPE
+    assert "grid 1" in _lib.last_error()
+    bad_lv = _lib.i32_array([4, 4, 3, 5])
+    assert create(ctypes.byref(plan), 0, 2, 2, ptrs, ext, bad_lv, mn, mx) in (
+        _lib.PALIWA_ERR_SHAPE, _lib.PALIWA_ERR_LEVEL)
+    assert create(ctypes.byref(plan), 0, 2, 0, ptrs, ext, lv, mn, mx) == _lib.PALIWA_ERR_INVALID_ARGUMENT
+    assert create(ctypes.byref(plan), 9, 2, 2, ptrs, ext, lv, mn, mx) == _lib.PALIWA_ERR_DTYPE
+    assert plan.value is None
+    assert _lib.lib.paliwa_batch_hierarchize(None, b"hat", None) == _lib.PALIWA_ERR_INVALID_ARGUMENT
+
+
+def test_combination_levels():
+    from paliwa_amd import combination_levels
+
+    # test/test_paliwa_combination_technique.cpp: the 4-D set of config 3
+    levels = combination_levels(4, (13, 14, 15, 16), 1)
+    assert len(levels) == 220 + 286 + 364 + 455
+    assert len(set(levels)) == len(levels)
+    assert all(min(l) >= 1 and sum(l) in (13, 14, 15, 16) for l in levels)

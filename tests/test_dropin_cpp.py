@@ -1,0 +1,27 @@
+"""Runs the C++ drop-in header test program (tests/cpp/test_dropin.cpp)."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "build", "test_dropin")
+
+
+def _binary():
+    if not os.path.exists(BIN):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "cpp")], check=True)
+    return BIN
+
+
+def test_dropin_header_exceptions_host_only():
+    out = subprocess.run([_binary(), "--host-only"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "host-only checks OK" in out.stdout
+
+
+@pytest.mark.gpu
+def test_dropin_header_on_gpu():
+    out = subprocess.run([_binary()], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "drop-in header checks OK" in out.stdout
