@@ -212,7 +212,7 @@ int paliwa_batch_create(paliwa_batch **plan, int32_t dtype, int32_t ndim,
       std::vector<BatchGrid> gd;
       std::vector<int32_t> tg;
       if (grp.fused) {
-        int64_t const tb = fused_tile_lines(dtype, grp.L);
+        int64_t const tb = fused_tile_lines(dtype, grp.L, grp.strided);
         for (size_t k = 0; k < members.size(); ++k) {
           int32_t const g = members[k];
           SweepParams const p =

@@ -1,50 +1,18 @@
-// float32 instantiations of the fused axis sweep (fused_sweep.hpp).
-// Chunk length K: 32 points (128 bytes) per lane; one lane per line for
-// periods up to 32, 64 points per lane for 4096.
+// float32 instantiations of the fused axis sweep (fused_sweep.hpp): line
+// periods 2^1 .. 2^12, chunk length from fused_chunk_of.
 #include "fused_sweep.hpp"
 
 namespace paliwa_hip {
 
 hipError_t launch_fused_f32(SweepParams const &p, hipStream_t stream) {
-  using fused::launch_kg;
-  using T = float;
-  switch (p.L) {
-  case 1: return launch_kg<T, 2, 1>(p, stream);
-  case 2: return launch_kg<T, 4, 1>(p, stream);
-  case 3: return launch_kg<T, 8, 1>(p, stream);
-  case 4: return launch_kg<T, 16, 1>(p, stream);
-  case 5: return launch_kg<T, 32, 1>(p, stream);
-  case 6: return launch_kg<T, 32, 2>(p, stream);
-  case 7: return launch_kg<T, 32, 4>(p, stream);
-  case 8: return launch_kg<T, 32, 8>(p, stream);
-  case 9: return launch_kg<T, 32, 16>(p, stream);
-  case 10: return launch_kg<T, 32, 32>(p, stream);
-  case 11: return launch_kg<T, 32, 64>(p, stream);
-  case 12: return launch_kg<T, 64, 64>(p, stream);
-  default: return hipErrorInvalidValue;
-  }
+  return fused::dispatch_period<float, 1, 12>(p, stream);
 }
 
 hipError_t launch_fused_batched_f32(SweepParams const &p, bool strided,
                                     ManyGrids grids, int64_t tiles,
                                     hipStream_t stream) {
-  using fused::launch_batched_kg;
-  using T = float;
-  switch (p.L) {
-  case 1: return launch_batched_kg<T, 2, 1>(p, strided, grids, tiles, stream);
-  case 2: return launch_batched_kg<T, 4, 1>(p, strided, grids, tiles, stream);
-  case 3: return launch_batched_kg<T, 8, 1>(p, strided, grids, tiles, stream);
-  case 4: return launch_batched_kg<T, 16, 1>(p, strided, grids, tiles, stream);
-  case 5: return launch_batched_kg<T, 32, 1>(p, strided, grids, tiles, stream);
-  case 6: return launch_batched_kg<T, 32, 2>(p, strided, grids, tiles, stream);
-  case 7: return launch_batched_kg<T, 32, 4>(p, strided, grids, tiles, stream);
-  case 8: return launch_batched_kg<T, 32, 8>(p, strided, grids, tiles, stream);
-  case 9: return launch_batched_kg<T, 32, 16>(p, strided, grids, tiles, stream);
-  case 10: return launch_batched_kg<T, 32, 32>(p, strided, grids, tiles, stream);
-  case 11: return launch_batched_kg<T, 32, 64>(p, strided, grids, tiles, stream);
-  case 12: return launch_batched_kg<T, 64, 64>(p, strided, grids, tiles, stream);
-  default: return hipErrorInvalidValue;
-  }
+  return fused::dispatch_period_batched<float, 1, 12>(p, strided, grids,
+                                                       tiles, stream);
 }
 
 } // namespace paliwa_hip

@@ -41,6 +41,16 @@ namespace fused {
 
 constexpr int kThreads = 256;
 constexpr int kXcds = 8;
+#ifdef PALIWA_STRIDED_DIRECT
+constexpr bool kStridedDirect = true;
+#else
+constexpr bool kStridedDirect = false;
+#endif
+// Workgroup size (see fused_threads): along a strided axis a tile holds NT/G
+// columns and every row of the tile is one run of NT/G consecutive points in
+// memory; wider runs waste less of the cache lines at their ends (rows start
+// at any 8-byte offset when the inner extent is odd, as with periodic images).
+constexpr int threads_for(bool strided, int G) { return fused_threads(strided, G); }
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 
@@ -202,36 +212,47 @@ __device__ __forceinline__ void run_generic(T (&v)[K], int gbase, int gi,
 //   strided axis:    row-major,  point r of column c at r*TB + c + (r >> LOGK)
 // One pad word per chunk puts the K-strided register loads of the G lanes of
 // a line on distinct banks.
-template <int K, int G> struct TileGeom {
-  static constexpr int TB = kThreads / G;
+template <int K, int G, bool STRIDED> struct TileGeom {
+  static constexpr int NT = threads_for(STRIDED, G);
+  static constexpr int TB = NT / G;
   static constexpr int LOGK = ilog2(K);
   __host__ __device__ static int line_pitch(int m) { return (m + G) | 1; }
   __host__ __device__ static int lds_elems(int m, bool strided) {
-    if (strided && G == 1)
+    if (strided && (G == 1 || kStridedDirect))
       return 0; // columns go straight to registers
     return strided ? m * TB + G + 2 : TB * line_pitch(m);
   }
 };
 
 // Per-thread staging state of one tile.
+//   contiguous axis: the tile is `count` consecutive points from `src`; thread
+//     t moves points t, t+256, t+512, ... (the first K of them exist in every
+//     full tile; the (K+1)-th only for the first TB threads, with images).
+//   strided axis: thread t moves rows row0, row0+G, ..., row0+(K-1)G of its
+//     column (always inside the lattice) and, with images, row n (only the
+//     threads with row0 == 0).  `lane` points at row row0 of the column and
+//     `step` = G*B is the uniform distance between its rows, so the per-lane
+//     state is one pointer whatever K is.
 template <typename T> struct Stage {
-  T *src = nullptr;  // contiguous axis: first point of the tile
-  T *colp = nullptr; // strided axis: this thread's column (nullptr: none)
-  int64_t B = 1;     // strided axis: element stride of the column
-  int count = 0;     // contiguous axis: points in the tile
+  T *src = nullptr;   // contiguous: first point of the tile
+  T *lane = nullptr;  // strided: row row0 of this thread's column (or null)
+  int64_t step = 0;   // strided: G * B
+  int64_t image = 0;  // strided: offset of row n from `lane` (0: no image row)
+  int count = 0;      // contiguous: points in the tile
+  bool full = false;  // contiguous: the tile holds TB whole lines
 };
 
-// Issue every HBM load of one tile for this thread (K+1 per lane, all in
-// flight together).
+// Issue every HBM load of one tile for this thread (K+1 per lane at most,
+// all in flight together).
 template <typename T, int K, int G, bool STRIDED>
 __device__ __forceinline__ Stage<T> load_tile(SweepParams const &p,
                                               TileRef const &ref,
                                               T (&buf)[K + 1]) {
-  using Geom = TileGeom<K, G>;
+  using Geom = TileGeom<K, G, STRIDED>;
   constexpr int TB = Geom::TB;
-  constexpr int ITER = K + 1;
   int const tid = threadIdx.x;
   int const m = p.m;
+  int const n = 1 << p.L;
   T *const g = static_cast<T *>(ref.data);
   Stage<T> st;
   if constexpr (!STRIDED) {
@@ -240,11 +261,19 @@ __device__ __forceinline__ Stage<T> load_tile(SweepParams const &p,
     int const nl = static_cast<int>(left < TB ? left : TB);
     st.src = g + line0 * m;
     st.count = nl * m;
+    st.full = nl == TB;
+    T const *const s = st.src + tid;
+    if (st.full) {
 #pragma unroll
-    for (int i = 0; i < ITER; ++i) {
-      int const e = tid + i * kThreads;
-      if (e < st.count)
-        buf[i] = st.src[e];
+      for (int i = 0; i < K; ++i)
+        buf[i] = s[i * Geom::NT];
+      if (tid + K * Geom::NT < st.count)
+        buf[K] = s[K * Geom::NT];
+    } else {
+#pragma unroll
+      for (int i = 0; i < K + 1; ++i)
+        if (tid + i * Geom::NT < st.count)
+          buf[i] = s[i * Geom::NT];
     }
   } else {
     int const col = tid % TB;
@@ -253,14 +282,14 @@ __device__ __forceinline__ Stage<T> load_tile(SweepParams const &p,
     if (c0 < ref.A * ref.B) {
       int64_t const a = c0 / ref.B;
       int64_t const b = c0 - a * ref.B;
-      st.B = ref.B;
-      st.colp = g + a * m * ref.B + b;
+      st.lane = g + a * m * ref.B + b + row0 * ref.B;
+      st.step = int64_t(G) * ref.B;
+      st.image = (m != n && row0 == 0) ? int64_t(n) * ref.B : 0;
 #pragma unroll
-      for (int i = 0; i < ITER; ++i) {
-        int const r = row0 + i * G;
-        if (r < m)
-          buf[i] = st.colp[r * st.B];
-      }
+      for (int i = 0; i < K; ++i)
+        buf[i] = st.lane[i * st.step];
+      if (st.image != 0)
+        buf[K] = st.lane[st.image];
     }
   }
   return st;
@@ -268,7 +297,7 @@ __device__ __forceinline__ Stage<T> load_tile(SweepParams const &p,
 
 template <int K, int G, bool STRIDED>
 __device__ __forceinline__ int lds_index(int line, int r, int LP) {
-  using Geom = TileGeom<K, G>;
+  using Geom = TileGeom<K, G, STRIDED>;
   if constexpr (STRIDED)
     return r * Geom::TB + line + (r >> Geom::LOGK);
   else
@@ -281,15 +310,14 @@ __device__ __forceinline__ void stash_tile(SweepParams const &p,
                                            Stage<T> const &st,
                                            T const (&buf)[K + 1], T *lds,
                                            uint32_t magic_m, int LP) {
-  using Geom = TileGeom<K, G>;
-  constexpr int ITER = K + 1;
+  using Geom = TileGeom<K, G, STRIDED>;
   int const tid = threadIdx.x;
   int const m = p.m;
   if constexpr (!STRIDED) {
 #pragma unroll
-    for (int i = 0; i < ITER; ++i) {
-      int const e = tid + i * kThreads;
-      if (e < st.count) {
+    for (int i = 0; i < K + 1; ++i) {
+      int const e = tid + i * Geom::NT;
+      if ((st.full && i < K) || e < st.count) {
         int const l = __umulhi(static_cast<uint32_t>(e), magic_m);
         int const r = e - l * m;
         lds[lds_index<K, G, false>(l, r, LP)] = buf[i];
@@ -298,13 +326,12 @@ __device__ __forceinline__ void stash_tile(SweepParams const &p,
   } else {
     int const col = tid % Geom::TB;
     int const row0 = tid / Geom::TB;
-    if (st.colp != nullptr) {
+    if (st.lane != nullptr) {
 #pragma unroll
-      for (int i = 0; i < ITER; ++i) {
-        int const r = row0 + i * G;
-        if (r < m)
-          lds[lds_index<K, G, true>(col, r, LP)] = buf[i];
-      }
+      for (int i = 0; i < K; ++i)
+        lds[lds_index<K, G, true>(col, row0 + i * G, LP)] = buf[i];
+      if (st.image != 0)
+        lds[lds_index<K, G, true>(col, 1 << p.L, LP)] = buf[K];
     }
   }
 }
@@ -314,43 +341,47 @@ template <typename T, int K, int G, bool STRIDED>
 __device__ __forceinline__ void store_tile(SweepParams const &p,
                                            Stage<T> const &st, T const *lds,
                                            uint32_t magic_m, int LP) {
-  using Geom = TileGeom<K, G>;
-  constexpr int ITER = K + 1;
+  using Geom = TileGeom<K, G, STRIDED>;
   int const tid = threadIdx.x;
   int const m = p.m;
-  T out[ITER];
+  T out[K + 1];
   if constexpr (!STRIDED) {
 #pragma unroll
-    for (int i = 0; i < ITER; ++i) {
-      int const e = tid + i * kThreads;
-      if (e < st.count) {
+    for (int i = 0; i < K + 1; ++i) {
+      int const e = tid + i * Geom::NT;
+      if ((st.full && i < K) || e < st.count) {
         int const l = __umulhi(static_cast<uint32_t>(e), magic_m);
         int const r = e - l * m;
         out[i] = lds[lds_index<K, G, false>(l, r, LP)];
       }
     }
+    T *const s = st.src + tid;
+    if (st.full) {
 #pragma unroll
-    for (int i = 0; i < ITER; ++i) {
-      int const e = tid + i * kThreads;
-      if (e < st.count)
-        st.src[e] = out[i];
+      for (int i = 0; i < K; ++i)
+        s[i * Geom::NT] = out[i];
+      if (tid + K * Geom::NT < st.count)
+        s[K * Geom::NT] = out[K];
+    } else {
+#pragma unroll
+      for (int i = 0; i < K + 1; ++i)
+        if (tid + i * Geom::NT < st.count)
+          s[i * Geom::NT] = out[i];
     }
   } else {
     int const col = tid % Geom::TB;
     int const row0 = tid / Geom::TB;
-    if (st.colp != nullptr) {
+    if (st.lane != nullptr) {
 #pragma unroll
-      for (int i = 0; i < ITER; ++i) {
-        int const r = row0 + i * G;
-        if (r < m)
-          out[i] = lds[lds_index<K, G, true>(col, r, LP)];
-      }
+      for (int i = 0; i < K; ++i)
+        out[i] = lds[lds_index<K, G, true>(col, row0 + i * G, LP)];
+      if (st.image != 0)
+        out[K] = lds[lds_index<K, G, true>(col, 1 << p.L, LP)];
 #pragma unroll
-      for (int i = 0; i < ITER; ++i) {
-        int const r = row0 + i * G;
-        if (r < m)
-          st.colp[r * st.B] = out[i];
-      }
+      for (int i = 0; i < K; ++i)
+        st.lane[i * st.step] = out[i];
+      if (st.image != 0)
+        st.lane[st.image] = out[K];
     }
   }
 }
@@ -365,7 +396,7 @@ template <typename T, int K, int G, bool STRIDED, int MODE, int PAT, bool ASC,
 __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
                                             Src const &src, uint32_t magic_m,
                                             int64_t ntiles, T *lds) {
-  using Geom = TileGeom<K, G>;
+  using Geom = TileGeom<K, G, STRIDED>;
   constexpr int TB = Geom::TB;
   int const tid = threadIdx.x;
   int const m = p.m;
@@ -391,22 +422,25 @@ __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
   };
 
   int64_t t = xcd_tile(blockIdx.x, nwg);
-  if constexpr (STRIDED && G == 1) {
-    // One lane per line: each lane's column goes straight between HBM and
-    // registers; the TB lanes of a wavefront read TB consecutive columns.
+  if constexpr (STRIDED && (G == 1 || kStridedDirect)) {
+    // Lanes load their own chunks straight between HBM and registers: lane
+    // (li, gi) holds rows gi*K .. gi*K+K-1 of column li of the tile.  With
+    // one lane per line (G == 1) the TB lanes of a wavefront read TB
+    // consecutive columns of one row, which is already coalesced.
     struct Column {
-      T *p;
-      int64_t B;
+      T *p;      // row gi*K of this lane's column
+      int64_t B; // row stride
     };
     auto column = [&](int64_t tile) -> Column {
       if (tile >= ntiles)
         return {nullptr, 1};
       TileRef const ref = src(tile);
-      int64_t const col = ref.local * TB + tid;
+      int64_t const col = ref.local * TB + li;
       if (col >= ref.A * ref.B)
         return {nullptr, ref.B};
       int64_t const a = col / ref.B;
-      return {static_cast<T *>(ref.data) + a * m * ref.B + (col - a * ref.B),
+      return {static_cast<T *>(ref.data) + a * m * ref.B + (col - a * ref.B) +
+                  int64_t(gi) * K * ref.B,
               ref.B};
     };
     T buf[K];
@@ -432,7 +466,7 @@ __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
 #pragma unroll
         for (int j = 0; j < K; ++j)
           cur.p[j * cur.B] = v[j];
-        if (m != n)
+        if (m != n && gi == 0)
           cur.p[static_cast<int64_t>(n) * cur.B] = v[0];
       }
       cur = next;
@@ -468,7 +502,7 @@ __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
 }
 
 template <typename T, int K, int G, bool STRIDED, int PAT, bool ASC>
-__global__ __launch_bounds__(kThreads) void fused_ladder(SweepParams p,
+__global__ __launch_bounds__(threads_for(STRIDED, G)) void fused_ladder(SweepParams p,
                                                          uint32_t magic_m,
                                                          int64_t ntiles) {
   extern __shared__ unsigned char smem_raw[];
@@ -480,7 +514,7 @@ __global__ __launch_bounds__(kThreads) void fused_ladder(SweepParams p,
 // The same sweep over the tiles of many grids with equal line length along
 // the axis (one launch of a batched transform).
 template <typename T, int K, int G, bool STRIDED, int PAT, bool ASC>
-__global__ __launch_bounds__(kThreads) void fused_ladder_batched(
+__global__ __launch_bounds__(threads_for(STRIDED, G)) void fused_ladder_batched(
     SweepParams p, uint32_t magic_m, int64_t ntiles, ManyGrids grids) {
   extern __shared__ unsigned char smem_raw[];
   sweep_tiles<T, K, G, STRIDED, 0, PAT, ASC>(p, grids, magic_m, ntiles,
@@ -488,7 +522,7 @@ __global__ __launch_bounds__(kThreads) void fused_ladder_batched(
 }
 
 template <typename T, int K, int G, bool STRIDED>
-__global__ __launch_bounds__(kThreads) void fused_generic(SweepParams p,
+__global__ __launch_bounds__(threads_for(STRIDED, G)) void fused_generic(SweepParams p,
                                                           uint32_t magic_m,
                                                           int64_t ntiles) {
   extern __shared__ unsigned char smem_raw[];
@@ -499,11 +533,12 @@ __global__ __launch_bounds__(kThreads) void fused_generic(SweepParams p,
 
 // Number of workgroups of `kernel` the device holds at once (all CUs), cached
 // per (kernel, LDS size): the persistent grid.
-int resident_workgroups(const void *kernel, size_t lds_bytes);
+int resident_workgroups(const void *kernel, size_t lds_bytes, int threads);
 
 template <typename Kernel, typename... Extra>
 hipError_t launch_one(Kernel kernel, SweepParams const &p, size_t bytes,
-                      int64_t tiles, hipStream_t stream, Extra... extra) {
+                      int64_t tiles, int threads, hipStream_t stream,
+                      Extra... extra) {
   if (bytes > 65536) {
     hipError_t err = hipFuncSetAttribute(
         reinterpret_cast<const void *>(kernel),
@@ -512,19 +547,19 @@ hipError_t launch_one(Kernel kernel, SweepParams const &p, size_t bytes,
       return err;
   }
   int64_t const resident =
-      resident_workgroups(reinterpret_cast<const void *>(kernel), bytes);
+      resident_workgroups(reinterpret_cast<const void *>(kernel), bytes, threads);
   uint32_t const grid =
       static_cast<uint32_t>(tiles < resident ? tiles : resident);
   // e / m by multiply-high: exact while (points per tile) * m < 2^32.
   uint32_t const magic = static_cast<uint32_t>(0xffffffffu / p.m + 1);
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), bytes, stream, p,
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), bytes, stream, p,
                      magic, tiles, extra...);
   return hipGetLastError();
 }
 
 template <typename T, int K, int G, bool STRIDED>
 hipError_t launch_kg_layout(SweepParams const &p, hipStream_t stream) {
-  using Geom = TileGeom<K, G>;
+  using Geom = TileGeom<K, G, STRIDED>;
   int64_t const lines = p.A * p.B;
   int64_t const tiles = (lines + Geom::TB - 1) / Geom::TB;
   size_t const bytes = sizeof(T) * Geom::lds_elems(p.m, STRIDED);
@@ -532,19 +567,19 @@ hipError_t launch_kg_layout(SweepParams const &p, hipStream_t stream) {
     bool const asc = p.schedule == kScheduleLadderAsc;
     switch (p.pattern) {
     case kPatOdd:
-      return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatOdd, true>, p, bytes, tiles, stream)
-                 : launch_one(fused_ladder<T, K, G, STRIDED, kPatOdd, false>, p, bytes, tiles, stream);
+      return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatOdd, true>, p, bytes, tiles, Geom::NT, stream)
+                 : launch_one(fused_ladder<T, K, G, STRIDED, kPatOdd, false>, p, bytes, tiles, Geom::NT, stream);
     case kPatOddEven:
-      return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatOddEven, true>, p, bytes, tiles, stream)
-                 : launch_one(fused_ladder<T, K, G, STRIDED, kPatOddEven, false>, p, bytes, tiles, stream);
+      return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatOddEven, true>, p, bytes, tiles, Geom::NT, stream)
+                 : launch_one(fused_ladder<T, K, G, STRIDED, kPatOddEven, false>, p, bytes, tiles, Geom::NT, stream);
     case kPatEvenOdd:
-      return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatEvenOdd, true>, p, bytes, tiles, stream)
-                 : launch_one(fused_ladder<T, K, G, STRIDED, kPatEvenOdd, false>, p, bytes, tiles, stream);
+      return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatEvenOdd, true>, p, bytes, tiles, Geom::NT, stream)
+                 : launch_one(fused_ladder<T, K, G, STRIDED, kPatEvenOdd, false>, p, bytes, tiles, Geom::NT, stream);
     default:
       break;
     }
   }
-  return launch_one(fused_generic<T, K, G, STRIDED>, p, bytes, tiles, stream);
+  return launch_one(fused_generic<T, K, G, STRIDED>, p, bytes, tiles, Geom::NT, stream);
 }
 
 template <typename T, int K, int G>
@@ -558,21 +593,21 @@ hipError_t launch_kg(SweepParams const &p, hipStream_t stream) {
 template <typename T, int K, int G, bool STRIDED>
 hipError_t launch_batched_layout(SweepParams const &p, ManyGrids grids,
                                  int64_t tiles, hipStream_t stream) {
-  using Geom = TileGeom<K, G>;
+  using Geom = TileGeom<K, G, STRIDED>;
   size_t const bytes = sizeof(T) * Geom::lds_elems(p.m, STRIDED);
   bool const asc = p.schedule == kScheduleLadderAsc;
   if (p.schedule != kScheduleLadderAsc && p.schedule != kScheduleLadderDesc)
     return hipErrorInvalidValue;
   switch (p.pattern) {
   case kPatOdd:
-    return asc ? launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOdd, true>, p, bytes, tiles, stream, grids)
-               : launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOdd, false>, p, bytes, tiles, stream, grids);
+    return asc ? launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOdd, true>, p, bytes, tiles, Geom::NT, stream, grids)
+               : launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOdd, false>, p, bytes, tiles, Geom::NT, stream, grids);
   case kPatOddEven:
-    return asc ? launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOddEven, true>, p, bytes, tiles, stream, grids)
-               : launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOddEven, false>, p, bytes, tiles, stream, grids);
+    return asc ? launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOddEven, true>, p, bytes, tiles, Geom::NT, stream, grids)
+               : launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOddEven, false>, p, bytes, tiles, Geom::NT, stream, grids);
   case kPatEvenOdd:
-    return asc ? launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatEvenOdd, true>, p, bytes, tiles, stream, grids)
-               : launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatEvenOdd, false>, p, bytes, tiles, stream, grids);
+    return asc ? launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatEvenOdd, true>, p, bytes, tiles, Geom::NT, stream, grids)
+               : launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatEvenOdd, false>, p, bytes, tiles, Geom::NT, stream, grids);
   default:
     return hipErrorInvalidValue;
   }
@@ -584,6 +619,37 @@ hipError_t launch_batched_kg(SweepParams const &p, bool strided,
                              hipStream_t stream) {
   return strided ? launch_batched_layout<T, K, G, true>(p, grids, tiles, stream)
                  : launch_batched_layout<T, K, G, false>(p, grids, tiles, stream);
+}
+
+// Compile-time dispatch on the line period 2^L (1 <= L <= LMAX).
+template <typename T, int L, int LMAX>
+hipError_t dispatch_period(SweepParams const &p, hipStream_t stream) {
+  if constexpr (L > LMAX) {
+    return hipErrorInvalidValue;
+  } else {
+    if (p.L == L) {
+      constexpr int K = fused_chunk_of(sizeof(T), L);
+      return launch_kg<T, K, (1 << L) / K>(p, stream);
+    }
+    return dispatch_period<T, L + 1, LMAX>(p, stream);
+  }
+}
+
+template <typename T, int L, int LMAX>
+hipError_t dispatch_period_batched(SweepParams const &p, bool strided,
+                                   ManyGrids grids, int64_t tiles,
+                                   hipStream_t stream) {
+  if constexpr (L > LMAX) {
+    return hipErrorInvalidValue;
+  } else {
+    if (p.L == L) {
+      constexpr int K = fused_chunk_of(sizeof(T), L);
+      return launch_batched_kg<T, K, (1 << L) / K>(p, strided, grids, tiles,
+                                                   stream);
+    }
+    return dispatch_period_batched<T, L + 1, LMAX>(p, strided, grids, tiles,
+                                                   stream);
+  }
 }
 
 } // namespace fused

@@ -100,17 +100,39 @@ hipError_t launch_fused_f32(SweepParams const &p, hipStream_t stream);
 
 // Points per lane (chunk length K) of the fused kernel for a line of period
 // 2^L; the kernel's tile holds 256*K / 2^L lines.
-inline int fused_chunk(int dtype, int L) {
-  int const target = dtype == PALIWA_FLOAT64 ? 16 : 32; // 128 bytes per lane
+#ifndef PALIWA_CHUNK_BYTES
+#define PALIWA_CHUNK_BYTES 128
+#endif
+constexpr int fused_chunk_of(int elem_bytes, int L) {
+  int const target = PALIWA_CHUNK_BYTES / elem_bytes;
   int const n = 1 << L;
-  if (n <= target)
-    return n;
-  if (n / target <= 64)
-    return target;
-  return n / 64;
+  return n <= target ? n : (n / target <= 64 ? target : n / 64);
 }
-inline int fused_tile_lines(int dtype, int L) {
-  return 256 * fused_chunk(dtype, L) / (1 << L);
+inline int fused_chunk(int dtype, int L) {
+  return fused_chunk_of(dtype == PALIWA_FLOAT64 ? 8 : 4, L);
+}
+// Workgroup size of the fused kernel; wider workgroups along strided axes
+// give each tile row a longer run of consecutive points.
+#ifndef PALIWA_STRIDED_THREADS
+#define PALIWA_STRIDED_THREADS 512
+#endif
+#ifndef PALIWA_STRIDED_THREADS_LONG
+#define PALIWA_STRIDED_THREADS_LONG PALIWA_STRIDED_THREADS
+#endif
+#ifndef PALIWA_CONTIG_THREADS
+#define PALIWA_CONTIG_THREADS 256
+#endif
+constexpr int fused_threads(bool strided, int G) {
+  if (!strided)
+    return G >= 4 ? PALIWA_CONTIG_THREADS : 256;
+  if (G >= 32)
+    return PALIWA_STRIDED_THREADS_LONG;
+  return G >= 4 ? PALIWA_STRIDED_THREADS : 256;
+}
+// Lines per tile of the fused kernel.
+inline int fused_tile_lines(int dtype, int L, bool strided) {
+  int const G = (1 << L) / fused_chunk(dtype, L);
+  return fused_threads(strided, G) / G;
 }
 
 // Batched fused sweep: `tiles` tiles of the grids in `grids`, all with line

@@ -205,13 +205,14 @@ void classify(SweepParams &p) {
 
 namespace fused {
 
-int resident_workgroups(const void *kernel, size_t lds_bytes) {
+int resident_workgroups(const void *kernel, size_t lds_bytes, int threads) {
   static std::mutex mutex;
   static std::map<std::pair<int, std::pair<const void *, size_t>>, int> cache;
   int device = 0;
   if (hipGetDevice(&device) != hipSuccess)
     return 1024;
   auto const key = std::make_pair(device, std::make_pair(kernel, lds_bytes));
+  // a kernel is always launched with the same block size
   {
     std::lock_guard<std::mutex> lock(mutex);
     auto it = cache.find(key);
@@ -219,7 +220,7 @@ int resident_workgroups(const void *kernel, size_t lds_bytes) {
       return it->second;
   }
   int per_cu = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kThreads,
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads,
                                                    lds_bytes) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
@@ -236,7 +237,7 @@ int resident_workgroups(const void *kernel, size_t lds_bytes) {
 } // namespace fused
 
 int fused_max_period(int dtype) {
-  return dtype == PALIWA_FLOAT64 ? 2048 : 4096;
+  return dtype == PALIWA_FLOAT64 ? 2048 : 4096; // the instantiated periods
 }
 
 int lds_line_max_period(int dtype) {
