@@ -98,6 +98,54 @@ def cpu_baseline(grid_cpu, budget_s):
     }
 
 
+def fp32_variant(torch, pw, original64, steps, warmup):
+    """The same workload with float32 storage and arithmetic: throughput and
+    error against float64 (max-rel and L2 of the coefficients, mass = the
+    coarse coefficient times the lattice size against the float64 lattice
+    sum, round trip)."""
+    level, lmin = list(LEVEL), [0] * len(LEVEL)
+    g32 = original64.to(torch.float32)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        pw.hierarchize(g32, level, lmin, level, WAVELET)
+        pw.dehierarchize(g32, level, lmin, level, WAVELET)
+
+    for _ in range(warmup):
+        step()
+    g32.copy_(original64)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    rt = (g32.double() - original64).abs().max().item() / original64.abs().max().item()
+    # coefficient errors against float64
+    c64 = original64.clone()
+    pw.hierarchize(c64, level, lmin, level, WAVELET)
+    c32 = original64.to(torch.float32)
+    pw.hierarchize(c32, level, lmin, level, WAVELET)
+    torch.cuda.synchronize()
+    d = c32.double() - c64
+    lat = original64[tuple(slice(0, 1 << l) for l in level)]
+    n_lat = lat.numel()
+    mass64 = lat.sum().item()
+    mass_rel = abs(c32[0, 0, 0].item() * n_lat - mass64) / lat.abs().sum().item()
+    n = original64.numel()
+    return {
+        "GBps": round(2 * 2 * len(level) * n * 4 / (ms * 1e-3) / 1e9, 1),
+        "ms_per_step": round(ms, 4),
+        "max_rel_err_vs_f64": d.abs().max().item() / c64.abs().max().item(),
+        "l2_rel_err_vs_f64": (d.norm() / c64.norm()).item(),
+        "mass_rel_err": mass_rel,
+        "round_trip_rel_err": rt,
+    }
+
+
 def ct_benchmark(torch, pw, fields, reps=5):
     """Combination-technique set: 1325 4-D float64 grids, |l|_1 in 13..16,
     l_i >= 1, hat basis, periodic images; cosine-product fields (seed 2).
@@ -322,6 +370,7 @@ def main():
         },
         "cpu_baseline": None,
     }
+    record["fp32"] = fp32_variant(torch, pw, original, args.steps, args.warmup)
     if not args.no_ct:
         record["ct"] = ct_benchmark(torch, pw, fields)
     if not args.no_cpu_baseline:

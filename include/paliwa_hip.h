@@ -4,8 +4,9 @@
  * (PALIWA_OK or a negative PALIWA_ERR_* code with a message available from
  * paliwa_last_error()), is stream-ordered and asynchronous on `stream` (a
  * hipStream_t; NULL is the default stream), never synchronizes the device and
- * allocates no device memory.  Arguments are validated on the host before any
- * HIP call, so a rejected call has no effect on the device.
+ * allocates no device memory (paliwa_batch_create alone allocates, once, the
+ * descriptor tables of its plan).  Arguments are validated on the host before
+ * any HIP call, so a rejected call has no effect on the device.
  *
  * Data layout: a dense row-major array (last axis contiguous, as DDC's
  * layout_right; paliwa/paliwa_distributed_transform.hpp:184), in device
