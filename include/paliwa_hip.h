@@ -183,6 +183,135 @@ int paliwa_batch_info(const paliwa_batch *plan, int32_t *launches,
 
 int paliwa_batch_destroy(paliwa_batch *plan);
 
+/* ---- Combination-technique combine -------------------------------------- *
+ * The steps after the batched hierarchize in the reference's combination
+ * technique (test/test_paliwa_combination_technique.cpp:176-313): subspace
+ * enumeration (paliwa/paliwa_utils.hpp:9-25), collection of the component
+ * grids onto sparse-grid storage, scatter onto the full grid at the maximum
+ * level.
+ *
+ * Grid g (dense row-major, extent 2^l or 2^l+1 per axis, dtype of the plan)
+ * stores, at index j along axis d, the point j * 2^(maximum_level[d] - l_d)
+ * of the full grid (paliwa/paliwa_domains.hpp:33-49).  Subspace k (0 <= k_d
+ * <= l_d) holds the points (2m+1) 2^(M_d - k_d), m < 2^(k_d - 1), along each
+ * axis with k_d >= 1 and the point 0 where k_d = 0
+ * (paliwa/paliwa_domains.hpp:51-83).  The plan keeps the subspaces contained
+ * in at least `min_count` grids (the reference keeps count > 1, i.e.
+ * min_count = 2; 1 keeps every subspace), in lexicographic order of k, each
+ * stored row-major over its points.  Sparse value at a point: the fold over
+ * the grids containing its subspace, in grid order, of
+ * s += coefficients[g] * grid_g (product then sum, each rounded; for float32
+ * grids both in double and the sum stored as float).
+ * Creating a plan validates the grids and allocates its tables (the only
+ * allocation); the grids' memory must stay valid while the plan is used. */
+#define PALIWA_COMBINE_MAX_DIMS 8
+typedef struct paliwa_combine paliwa_combine;
+
+int paliwa_combine_create(paliwa_combine **plan, int32_t dtype, int32_t ndim,
+                          int32_t ngrids, void *const *data,
+                          const int64_t *extents, const int32_t *levels,
+                          const double *coefficients,
+                          const int32_t *maximum_level, int32_t min_count);
+/* Kept subspaces, sparse points, multiply-adds of a gather, table bytes. */
+int paliwa_combine_info(const paliwa_combine *plan, int32_t *subspaces,
+                        int64_t *sparse_points, int64_t *terms,
+                        int64_t *table_bytes);
+/* Host copies: levels [subspaces][ndim], offsets [subspaces + 1], counts
+ * [subspaces] (grids containing each subspace); any may be NULL. */
+int paliwa_combine_subspaces(const paliwa_combine *plan, int32_t *levels,
+                             int64_t *offsets, int32_t *counts);
+/* sparse (device, sparse_points elements) := the combined subspaces. */
+int paliwa_combine_gather(paliwa_combine *plan, void *sparse, void *stream);
+/* full (device, extent 2^maximum_level or 2^maximum_level+1 per axis; an
+ * image plane repeats index 0) := 0 + sparse value at the points of kept
+ * subspaces, 0 elsewhere. */
+int paliwa_combine_scatter(const paliwa_combine *plan, const void *sparse,
+                           void *full, const int64_t *full_extents,
+                           void *stream);
+int paliwa_combine_destroy(paliwa_combine *plan);
+
+/* ---- Learned lifting (no counterpart in the reference) ------------------ *
+ * An extension on top of the plain method: an integer lifting transform of an
+ * int32 fixed-point grid whose predict step at levels >= 3 adds to the
+ * interpolet (hat) prediction  round(clamp(unit * mlp(u), +-bound * unit)),
+ * from c = the 4 coarse neighbours on each side along the lifted axis:
+ *   features 0 (raw):       u = c / max|c|, unit = max|c|, bound 4;
+ *   features 1 (detrended): D_k = 2 c_k - (c_3 + c_4) - (c_4 - c_3)(2k - 7),
+ *                           the deviation from the interpolet line (times 2),
+ *                           u = D / max|D|, unit = max|D| / 2, bound 1;
+ * mlp = 8 -> 32 -> 32 -> 1 with GELU, bf16 weights and activations and fp32
+ * accumulation on MFMA.  Integer lifting makes every step exactly invertible
+ * (the inverse recomputes the bit-identical prediction), so the only round-trip
+ * error is the quantization paliwa_quantize_f32 applies.  `update` = 1 adds the
+ * integer CDF 5/3 update after each predict.  mlp_mask[d]: bit l set = level l
+ * of axis d uses the MLP (else the interpolet, the fallback).  Forward axes run
+ * 0..ndim-1, inverse ndim-1..0.  Weights: paliwa_learned_param_count() fp32 in
+ * device memory, packed W1[32][8], b1[32], W2[32][32], b2[32], w3[32], b3.
+ * Lines of at most 2^11 points.  `overflow` (device int32, may be NULL) is set
+ * to 1 when an intermediate leaves the int32 range (the result is then not
+ * invertible). */
+int paliwa_learned_param_count(void);
+
+/* out[i] = round(in[i] / scale) (int32); in[i] = out[i] * scale. */
+int paliwa_quantize_f32(const float *in, int32_t *out, int64_t count,
+                        float scale, void *stream);
+int paliwa_dequantize_f32(const int32_t *in, float *out, int64_t count,
+                          float scale, void *stream);
+
+/* Forward transform along `axis` (or every axis, 0..ndim-1, for axis = -1).
+ * features: 0 = the neighbours divided by their max-abs; 1 = the neighbours'
+ * deviations from the interpolet line through the two nearest ones, divided
+ * by their max-abs.  `stats` (device double[ndim][2*(PALIWA_MAX_LEVEL+1)], may
+ * be NULL) accumulates, per axis and MLP level, sum |detail| with the MLP
+ * prediction ([2l]) and with the interpolet ([2l+1]). */
+int paliwa_learned_hierarchize(int32_t *data, int32_t ndim,
+                               const int64_t *extents, const int32_t *level,
+                               const int32_t *minimum_level, int32_t axis,
+                               const float *weights, const uint32_t *mlp_mask,
+                               int32_t update, int32_t features, double *stats,
+                               int32_t *overflow, void *stream);
+/* Inverse along `axis` (or every axis, ndim-1..0, for axis = -1). */
+int paliwa_learned_dehierarchize(int32_t *data, int32_t ndim,
+                                 const int64_t *extents, const int32_t *level,
+                                 const int32_t *minimum_level, int32_t axis,
+                                 const float *weights,
+                                 const uint32_t *mlp_mask, int32_t update,
+                                 int32_t features, int32_t *overflow,
+                                 void *stream);
+
+/* Training pairs from the interpolet transform of `data` (transformed in
+ * place): at every axis and level >= 3, a sampled subset (hash below
+ * sample_threshold, out of 2^32) of the predicted points writes
+ * {8 network inputs, target (x - interpolet) / unit clamped to +-bound, key}
+ * (10 floats) to pairs[*pair_count], up to `capacity` pairs. */
+int paliwa_learned_extract_pairs(int32_t *data, int32_t ndim,
+                                 const int64_t *extents, const int32_t *level,
+                                 const int32_t *minimum_level, int32_t update,
+                                 int32_t features, float *pairs,
+                                 uint32_t *pair_count, uint32_t capacity,
+                                 uint32_t sample_threshold, uint32_t salt,
+                                 void *stream);
+
+/* The network on inputs[count][8] through the MFMA path of the transform. */
+int paliwa_learned_mlp_eval(const float *inputs, const float *weights,
+                            float *out, int32_t count, void *stream);
+
+/* Adds to grad the gradient of mean_i (mlp(u_i) - t_i)^2 over rows
+ * index[0..count) (or 0..count) of pairs (row stride `stride` floats: 8
+ * inputs then the target), and the loss to *loss_sum. */
+int paliwa_learned_gradient(const float *pairs, int32_t stride,
+                            const int32_t *index, int32_t count,
+                            const float *weights, float *grad,
+                            float *loss_sum, void *stream);
+
+/* One Adam step (step = 1, 2, ...) on the fp32 weights; clears grad. */
+int paliwa_learned_adam(float *weights, float *grad, float *m1, float *m2,
+                        float lr, float beta1, float beta2, float eps,
+                        int32_t step, void *stream);
+
+/* p[0..count) = value. */
+int paliwa_learned_fill(float *p, int32_t count, float value, void *stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -23,6 +23,7 @@ from .transform import (  # noqa: F401
     wavelet_filters,
 )
 from .batch import CombinationBatch, combination_levels  # noqa: F401
+from .combine import SparseGridCombination  # noqa: F401
 from ._lib import LIBRARY_PATH  # noqa: F401
 
 __version__ = "0.1.0"

@@ -86,6 +86,31 @@ SIGNATURES = {
     "paliwa_batch_set_graphs": (ctypes.c_int, [_vp, _i32]),
     "paliwa_batch_info": (ctypes.c_int, [_vp, _i32p, _i64p, _i64p]),
     "paliwa_batch_destroy": (ctypes.c_int, [_vp]),
+    "paliwa_combine_create": (
+        ctypes.c_int, [ctypes.POINTER(_vp), _i32, _i32, _i32, ctypes.POINTER(_vp), _i64p,
+                       _i32p, ctypes.POINTER(ctypes.c_double), _i32p, _i32]),
+    "paliwa_combine_info": (ctypes.c_int, [_vp, _i32p, _i64p, _i64p, _i64p]),
+    "paliwa_combine_subspaces": (ctypes.c_int, [_vp, _i32p, _i64p, _i32p]),
+    "paliwa_combine_gather": (ctypes.c_int, [_vp, _vp, _vp]),
+    "paliwa_combine_scatter": (ctypes.c_int, [_vp, _vp, _vp, _i64p, _vp]),
+    "paliwa_combine_destroy": (ctypes.c_int, [_vp]),
+    "paliwa_learned_param_count": (ctypes.c_int, []),
+    "paliwa_quantize_f32": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_float, _vp]),
+    "paliwa_dequantize_f32": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_float, _vp]),
+    "paliwa_learned_hierarchize": (
+        ctypes.c_int, [_vp, _i32, _i64p, _i32p, _i32p, _i32, _vp, _vp, _i32, _i32, _vp, _vp,
+                       _vp]),
+    "paliwa_learned_dehierarchize": (
+        ctypes.c_int, [_vp, _i32, _i64p, _i32p, _i32p, _i32, _vp, _vp, _i32, _i32, _vp, _vp]),
+    "paliwa_learned_extract_pairs": (
+        ctypes.c_int, [_vp, _i32, _i64p, _i32p, _i32p, _i32, _i32, _vp, _vp, ctypes.c_uint32,
+                       ctypes.c_uint32, ctypes.c_uint32, _vp]),
+    "paliwa_learned_mlp_eval": (ctypes.c_int, [_vp, _vp, _vp, _i32, _vp]),
+    "paliwa_learned_gradient": (ctypes.c_int, [_vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp]),
+    "paliwa_learned_adam": (
+        ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                       ctypes.c_float, _i32, _vp]),
+    "paliwa_learned_fill": (ctypes.c_int, [_vp, _i32, ctypes.c_float, _vp]),
 }
 
 

@@ -97,9 +97,9 @@ hipGraphExec_t capture(paliwa_batch &b, WaveletEntry const &w, bool forward) {
         exec = nullptr;
     }
     if (graph != nullptr)
-      hipGraphDestroy(graph);
+      (void)hipGraphDestroy(graph);
   }
-  hipStreamDestroy(cs);
+  (void)hipStreamDestroy(cs);
   (void)hipGetLastError();
   return exec;
 }
@@ -146,9 +146,9 @@ int execute(paliwa_batch *b, char const *wavelet_name, bool forward,
 
 void release(paliwa_batch *b) {
   for (auto &kv : b->graphs)
-    hipGraphExecDestroy(kv.second);
+    (void)hipGraphExecDestroy(kv.second);
   if (b->dev != nullptr)
-    hipFree(b->dev);
+    (void)hipFree(b->dev);
   delete b;
 }
 

@@ -1,0 +1,796 @@
+// Learned lifting (an extension with no counterpart in the reference): an
+// integer lifting transform whose predict step at levels >= 3 adds to the
+// interpolet (hat) predict the output of a small MLP
+//     8 -> 32 -> 32 -> 1, GELU, bf16 weights and activations, fp32 accumulate,
+// evaluated with bf16 MFMA (v_mfma_f32_16x16x16_bf16), whose inputs are the 4
+// coarse neighbours on each side of the predicted point along the lifted
+// axis, divided by their max-abs.  The interpolet predict is the fallback.
+//
+// Why integers: a prediction that passes through bf16 roundings is a
+// discontinuous function of its inputs, so a floating-point lifting would
+// reconstruct coarse values that differ from the forward ones in the last bit
+// and the inverse prediction could then differ by a bf16 step.  On integers
+// (fixed point: q = round(f / scale)) every lifting step is exact
+// (Calderbank-Daubechies-Sweldens-Yeo integer lifting): the inverse sees
+// bit-identical coarse values, recomputes the bit-identical prediction (same
+// kernel, same instruction sequence, deterministic MFMA) and subtracts it back.
+// The only round-trip error is the quantization, |f - scale*q| <= scale/2.
+//
+// One axis sweep (levels l = L .. lmin+1 forward, reverse order inverse; the
+// storage stride of level l is s = 2^(L-l), period n = 2^L):
+//   predict (odd points x = s (mod 2s)), with coarse neighbours
+//     c[-4..-1, 1..4] = g[x -/+ s, 3s, 5s, 7s] (periodic):
+//       P = floor((c[-1] + c[1] + 1) / 2)                  interpolet
+//         + round(sigma * mlp(c / sigma))                  levels >= 3 when on
+//       (sigma = max |c|; the MLP term is clamped to +-4 sigma)
+//       forward g[x] -= P,  inverse g[x] += P
+//   update (even points, when enabled; the integer CDF 5/3 update):
+//       forward g[e] += floor((g[e-s] + g[e+s] + 2) / 4), inverse g[e] -= ...
+// Forward: predict then update per level, finest level first.  Inverse: undo
+// update then predict per level, coarsest first.  Axes: forward 0..D-1,
+// inverse D-1..0 (the learned steps of different axes do not commute).
+#include "common.hpp"
+#include "lifting.hpp"
+
+#include <cmath>
+#include <cstdint>
+
+namespace paliwa_hip {
+namespace learned {
+
+constexpr int kThreads = 256;
+constexpr int kTile = 16;    // lines per tile
+constexpr int kHidden = 32;
+constexpr int kInputs = 8;
+// Packed fp32 parameter layout.
+constexpr int kW1 = 0;                         // [32][8]
+constexpr int kB1 = kW1 + kHidden * kInputs;   // [32]
+constexpr int kW2 = kB1 + kHidden;             // [32][32]
+constexpr int kB2 = kW2 + kHidden * kHidden;   // [32]
+constexpr int kW3 = kB2 + kHidden;             // [32]
+constexpr int kB3 = kW3 + kHidden;             // [1]
+constexpr int kParams = kB3 + 1;               // 1377
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// fp32 -> bf16 bits, round to nearest even (finite inputs).
+__device__ __forceinline__ short to_bf16(float x) {
+  uint32_t const u = __float_as_uint(x);
+  uint32_t const r = u + 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<short>(r >> 16);
+}
+__device__ __forceinline__ float from_bf16(short b) {
+  return __uint_as_float(static_cast<uint32_t>(static_cast<uint16_t>(b)) << 16);
+}
+__device__ __forceinline__ float round_bf16(float x) {
+  return from_bf16(to_bf16(x));
+}
+
+// GELU, tanh form.
+__device__ __forceinline__ float gelu(float z) {
+  float const t = tanhf(0.7978845608028654f * (z + 0.044715f * z * z * z));
+  return 0.5f * z * (1.0f + t);
+}
+__device__ __forceinline__ float gelu_grad(float z) {
+  float const a = 0.7978845608028654f * (z + 0.044715f * z * z * z);
+  float const t = tanhf(a);
+  float const da = 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * z * z);
+  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * da;
+}
+
+// Per-lane register fragments of the network for the MFMA evaluation.
+// Transposed formulation: each layer computes H^T = W * X^T with A = weight
+// rows (units) x K and B = X^T (K x 16 points), so the D layout of one layer
+// (lane l: units 4*(l/16)+j, point l%16) is the B layout of the next.
+struct Net {
+  bf16x4 w1[2];    // A fragments of [W1 | b1 | 0] (K padded 8 -> 16), 2 row blocks
+  bf16x4 w2[2][2]; // A fragments of W2, [row block][k block]
+  float b2[2][4];  // bias of the 4 units this lane holds per row block
+  float w3[2][4];  // output weights of those units (bf16 values)
+  float b3;
+};
+
+__device__ __forceinline__ void load_net(float const *prm, Net &net) {
+  int const lane = threadIdx.x & 63;
+  int const row = lane & 15;
+  int const kq = 4 * (lane >> 4);
+#pragma unroll
+  for (int blk = 0; blk < 2; ++blk) {
+    int const unit = 16 * blk + row;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      int const k = kq + j;
+      float w = 0.0f;
+      if (k < kInputs)
+        w = prm[kW1 + unit * kInputs + k];
+      else if (k == kInputs)
+        w = prm[kB1 + unit];
+      net.w1[blk][j] = to_bf16(w);
+    }
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        net.w2[blk][kb][j] = to_bf16(prm[kW2 + unit * kHidden + 16 * kb + kq + j]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      int const u = 16 * blk + kq + j; // D-layout unit of this lane
+      net.b2[blk][j] = prm[kB2 + u];
+      net.w3[blk][j] = round_bf16(prm[kW3 + u]);
+    }
+  }
+  net.b3 = prm[kB3];
+}
+
+// MLP on 16 points of one wavefront.  Lane l supplies inputs
+// x[4*(l/16) .. +3] of point l%16 (lanes 32..63: the bias input 1 at k = 8,
+// zeros elsewhere).  Returns the network output of point l%16 in every lane.
+__device__ __forceinline__ float mlp16(Net const &net, bf16x4 xin) {
+  f32x4 const zero = {0.0f, 0.0f, 0.0f, 0.0f};
+  bf16x4 h1[2];
+#pragma unroll
+  for (int blk = 0; blk < 2; ++blk) {
+    f32x4 const d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w1[blk], xin, zero, 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      h1[blk][j] = to_bf16(gelu(d[j]));
+  }
+  float partial = 0.0f;
+#pragma unroll
+  for (int blk = 0; blk < 2; ++blk) {
+    f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w2[blk][0], h1[0], zero, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w2[blk][1], h1[1], d, 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      partial += net.w3[blk][j] * round_bf16(gelu(d[j] + net.b2[blk][j]));
+  }
+  // sum over the 4 lanes holding the same point (a + b == b + a exactly, so
+  // every lane ends with the same bits)
+  partial += __shfl_xor(partial, 16, 64);
+  partial += __shfl_xor(partial, 32, 64);
+  return partial + net.b3;
+}
+
+// Inputs of this lane's 4 MLP slots for one predicted point: the 8 features
+// (lanes 0..31), the bias input 1 / zeros (lanes 32..63).
+__device__ __forceinline__ bf16x4 mlp_inputs(float const (&v)[8]) {
+  int const q = (threadIdx.x & 63) >> 4;
+  bf16x4 x;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    float const u = q < 2 ? v[4 * q + j] : ((q == 2 && j == 0) ? 1.0f : 0.0f);
+    x[j] = to_bf16(u);
+  }
+  return x;
+}
+
+// Network inputs of one predicted point from its 8 coarse neighbours
+// c[0..7] (positions -7s, -5s, ..., +7s).
+//   raw:       u_k = c_k / max|c|,                 correction = max|c| * r
+//   detrended: D_k = 2 c_k - (c_3 + c_4) - (c_4 - c_3)(2k - 7), twice the
+//              deviation of c_k from the interpolet line through c_3, c_4
+//              (exact in integers; D_3 = D_4 = 0), u_k = D_k / max|D|,
+//              correction = max|D| / 2 * r
+// Raw neighbours normalized by their max-abs carry only the ~2^-9 relative
+// resolution of bf16, below the curvature the interpolet misses on resolved
+// data; detrending removes the part the interpolet already predicts first.
+enum Features : int32_t { kFeaturesRaw = 0, kFeaturesDetrended = 1 };
+
+// Largest correction, in units of the feature kind's correction unit.  For
+// detrended inputs one unit (max|D| / 2) already exceeds the residual of any
+// point between two monotone runs of neighbours (a step between c_3 and c_4
+// gives |x - hat| <= max|D| / 12), and a tight bound limits the growth of
+// the details, hence of the coarse values, where the network is wrong.
+__device__ __forceinline__ float correction_bound(int32_t kind) {
+  return kind == kFeaturesDetrended ? 1.0f : 4.0f;
+}
+
+__device__ __forceinline__ float features(int const (&c)[8], int32_t kind,
+                                          float (&u)[8], int64_t &norm) {
+  int64_t f[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    f[k] = kind == kFeaturesDetrended
+               ? 2 * int64_t(c[k]) - (int64_t(c[3]) + c[4]) -
+                     (int64_t(c[4]) - c[3]) * (2 * k - 7)
+               : int64_t(c[k]);
+  norm = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    int64_t const v = f[k] < 0 ? -f[k] : f[k];
+    norm = v > norm ? v : norm;
+  }
+  float const s = static_cast<float>(norm);
+  float const inv = norm > 0 ? 1.0f / s : 0.0f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    u[k] = static_cast<float>(f[k]) * inv;
+  return kind == kFeaturesDetrended ? 0.5f * s : s; // correction unit
+}
+
+struct LearnedArgs {
+  int32_t *data;
+  int64_t A, B;
+  int32_t m, L;
+  int32_t lmin;
+  int32_t inverse;
+  int32_t update;
+  uint32_t mlp_mask;     // bit l: MLP predict at level l
+  int32_t features;      // kFeatures*
+  float const *weights;  // kParams fp32
+  double *stats;         // [L+1][2] sum |d| with MLP / interpolet, or null
+  // pair extraction (training data), or null
+  float *pairs;          // [capacity][10]: 8 inputs, target, key
+  uint32_t *pair_count;
+  uint32_t pair_capacity;
+  uint32_t sample_threshold; // keep a candidate when hash < threshold
+  uint32_t salt;
+  int32_t *overflow;     // set to 1 if an intermediate left the int32 range
+};
+
+__device__ __forceinline__ uint32_t mix(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  h *= 0x846ca68bu;
+  h ^= h >> 16;
+  return h;
+}
+
+template <bool STRIDED>
+__device__ __forceinline__ int lidx(int line, int r, int m) {
+  return STRIDED ? r * kTile + line : line * m + r;
+}
+
+template <bool STRIDED>
+__global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
+  extern __shared__ int32_t tile[];
+  __shared__ int32_t overflow_flag;
+  int const tid = threadIdx.x;
+  int const lane = tid & 63;
+  int const wave = tid >> 6;
+  int const m = a.m;
+  int const n = 1 << a.L;
+  int64_t const lines = a.A * a.B;
+  int64_t const line0 = int64_t(blockIdx.x) * kTile;
+  int const nl = static_cast<int>(lines - line0 < kTile ? lines - line0 : kTile);
+  if (tid == 0)
+    overflow_flag = 0;
+
+  Net net;
+  bool const any_mlp = (a.mlp_mask >> 3) != 0 || a.pairs != nullptr;
+  if (any_mlp && a.weights != nullptr)
+    load_net(a.weights, net);
+
+  // column / line base pointers
+  auto base_of = [&](int l) -> int32_t * {
+    int64_t const ln = line0 + l;
+    int64_t const aa = ln / a.B;
+    return a.data + aa * m * a.B + (ln - aa * a.B);
+  };
+  // load
+  for (int e = tid; e < kTile * m; e += kThreads) {
+    int const l = STRIDED ? e % kTile : e / m;
+    int const r = STRIDED ? e / kTile : e % m;
+    if (l < nl)
+      tile[lidx<STRIDED>(l, r, m)] = base_of(l)[int64_t(r) * a.B];
+  }
+  __syncthreads();
+
+  int const nlev = a.L - a.lmin;
+  for (int step = 0; step < nlev; ++step) {
+    int const lev = a.inverse ? a.lmin + 1 + step : a.L - step;
+    int const s = 1 << (a.L - lev);
+    int const cnt = n / (2 * s); // odd points per line
+    // inverse: undo the update first
+    if (a.update && a.inverse) {
+      for (int p = tid; p < nl * cnt; p += kThreads) {
+        int const l = p / cnt, k = p % cnt;
+        int const e = 2 * s * k;
+        int const lo = e == 0 ? n - s : e - s;
+        int64_t const u = (int64_t(tile[lidx<STRIDED>(l, lo, m)]) +
+                           tile[lidx<STRIDED>(l, e + s, m)] + 2) >> 2;
+        tile[lidx<STRIDED>(l, e, m)] -= static_cast<int32_t>(u);
+      }
+      __syncthreads();
+    }
+    bool const use_mlp = lev >= 3 && ((a.mlp_mask >> lev) & 1u);
+    bool const want_pairs = a.pairs != nullptr && lev >= 3;
+    bool const want_stats = a.stats != nullptr && lev >= 3 && use_mlp;
+    if (use_mlp || want_pairs || want_stats) {
+      // batches of 16 points per wavefront
+      int const batches = (nl * cnt + 15) / 16;
+      double l1_mlp = 0.0, l1_hat = 0.0;
+      for (int bt = wave; bt < batches; bt += kThreads / 64) {
+        int const p = bt * 16 + (lane & 15);
+        bool const valid = p < nl * cnt;
+        int const l = valid ? p / cnt : 0;
+        int const k = valid ? p % cnt : 0;
+        int const x = s + 2 * s * k;
+        int c[8];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          int const off = (2 * (3 - j) + 1) * s; // 7s, 5s, 3s, s
+          c[j] = tile[lidx<STRIDED>(l, ((x - off) % n + n) % n, m)];
+          c[7 - j] = tile[lidx<STRIDED>(l, (x + off) % n, m)];
+        }
+        float u[8];
+        int64_t norm = 0;
+        float const unit = features(c, a.features, u, norm);
+        float const r = a.weights != nullptr ? mlp16(net, mlp_inputs(u)) : 0.0f;
+        int64_t const hat = (int64_t(c[3]) + c[4] + 1) >> 1;
+        float corr = unit * r;
+        float const bound = correction_bound(a.features);
+        corr = fminf(fmaxf(corr, -bound * unit), bound * unit);
+        int64_t const pm = hat + static_cast<int64_t>(rintf(corr));
+        int32_t const xv = tile[lidx<STRIDED>(l, x, m)];
+        if (valid && lane < 16) {
+          if (want_pairs && !a.inverse) {
+            uint32_t const key = mix(static_cast<uint32_t>(line0 + l) * 2654435761u ^
+                                     mix(static_cast<uint32_t>(x) ^ (a.salt << 8) ^
+                                         static_cast<uint32_t>(lev)));
+            if (key < a.sample_threshold && norm > 0) {
+              uint32_t const slot = atomicAdd(a.pair_count, 1u);
+              if (slot < a.pair_capacity) {
+                float *pr = a.pairs + size_t(slot) * 10;
+                for (int j = 0; j < 8; ++j)
+                  pr[j] = u[j];
+                // the predictor clamps its correction to +-bound units, so
+                // targets beyond that cannot be fitted: clamp them too
+                pr[8] = fminf(fmaxf(static_cast<float>(int64_t(xv) - hat) / unit, -bound), bound);
+                pr[9] = __uint_as_float(key);
+              }
+            }
+          }
+          if (want_stats) {
+            l1_mlp += fabs(double(int64_t(xv) - pm));
+            l1_hat += fabs(double(int64_t(xv) - hat));
+          }
+          int64_t const pred = use_mlp ? pm : hat;
+          int64_t const out = a.inverse ? int64_t(xv) + pred : int64_t(xv) - pred;
+          if (out > INT32_MAX || out < INT32_MIN)
+            overflow_flag = 1;
+          tile[lidx<STRIDED>(l, x, m)] = static_cast<int32_t>(out);
+        }
+      }
+      if (want_stats) {
+        // lanes 16..63 hold zeros
+        atomicAdd(&a.stats[2 * lev + 0], l1_mlp);
+        atomicAdd(&a.stats[2 * lev + 1], l1_hat);
+      }
+    } else {
+      for (int p = tid; p < nl * cnt; p += kThreads) {
+        int const l = p / cnt, k = p % cnt;
+        int const x = s + 2 * s * k;
+        int const lo = x - s;
+        int const hi = x + s == n ? 0 : x + s;
+        int64_t const hat = (int64_t(tile[lidx<STRIDED>(l, lo, m)]) +
+                             tile[lidx<STRIDED>(l, hi, m)] + 1) >> 1;
+        int64_t const xv = tile[lidx<STRIDED>(l, x, m)];
+        int64_t const out = a.inverse ? xv + hat : xv - hat;
+        if (out > INT32_MAX || out < INT32_MIN)
+          overflow_flag = 1;
+        tile[lidx<STRIDED>(l, x, m)] = static_cast<int32_t>(out);
+      }
+    }
+    __syncthreads();
+    if (a.update && !a.inverse) {
+      for (int p = tid; p < nl * cnt; p += kThreads) {
+        int const l = p / cnt, k = p % cnt;
+        int const e = 2 * s * k;
+        int const lo = e == 0 ? n - s : e - s;
+        int64_t const u = (int64_t(tile[lidx<STRIDED>(l, lo, m)]) +
+                           tile[lidx<STRIDED>(l, e + s, m)] + 2) >> 2;
+        int64_t const out = int64_t(tile[lidx<STRIDED>(l, e, m)]) + u;
+        if (out > INT32_MAX || out < INT32_MIN)
+          overflow_flag = 1;
+        tile[lidx<STRIDED>(l, e, m)] = static_cast<int32_t>(out);
+      }
+      __syncthreads();
+    }
+  }
+  // periodic image
+  if (m != n)
+    for (int l = tid; l < nl; l += kThreads)
+      tile[lidx<STRIDED>(l, n, m)] = tile[lidx<STRIDED>(l, 0, m)];
+  __syncthreads();
+  for (int e = tid; e < kTile * m; e += kThreads) {
+    int const l = STRIDED ? e % kTile : e / m;
+    int const r = STRIDED ? e / kTile : e % m;
+    if (l < nl)
+      base_of(l)[int64_t(r) * a.B] = tile[lidx<STRIDED>(l, r, m)];
+  }
+  if (tid == 0 && overflow_flag != 0 && a.overflow != nullptr)
+    atomicOr(a.overflow, 1);
+}
+
+// ---------------------------------------------------------------------------
+// quantization
+
+__global__ void quantize_kernel(float const *in, int32_t *out, int64_t count,
+                                float inv_scale) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < count;
+       i += int64_t(gridDim.x) * blockDim.x)
+    out[i] = static_cast<int32_t>(rintf(in[i] * inv_scale));
+}
+
+__global__ void dequantize_kernel(int32_t const *in, float *out, int64_t count,
+                                  float scale) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < count;
+       i += int64_t(gridDim.x) * blockDim.x)
+    out[i] = static_cast<float>(in[i]) * scale;
+}
+
+// ---------------------------------------------------------------------------
+// MLP evaluation on given inputs (MFMA path, as inside the transform)
+
+__global__ __launch_bounds__(kThreads) void mlp_eval_kernel(
+    float const *inputs, float const *weights, float *out, int32_t count) {
+  Net net;
+  load_net(weights, net);
+  int const lane = threadIdx.x & 63;
+  int const wave_global = (blockIdx.x * kThreads + threadIdx.x) >> 6;
+  int const waves = (gridDim.x * kThreads) >> 6;
+  int const q = lane >> 4;
+  for (int b = wave_global; b * 16 < count; b += waves) {
+    int const p = b * 16 + (lane & 15);
+    bool const valid = p < count;
+    bf16x4 x;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float v;
+      if (q < 2)
+        v = valid ? inputs[size_t(p) * 8 + 4 * q + j] : 0.0f;
+      else
+        v = (q == 2 && j == 0) ? 1.0f : 0.0f;
+      x[j] = to_bf16(v);
+    }
+    float const r = mlp16(net, x);
+    if (valid && lane < 16)
+      out[p] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// training: loss and gradient of the mean squared error over a batch
+//   loss = mean_i (mlp(u_i) - t_i)^2
+// Forward in fp32 on bf16-rounded weights, inputs and activations (the
+// precision of the MFMA path); backward in fp32 through the rounded values.
+
+constexpr int kTrainThreads = 64;
+
+__global__ __launch_bounds__(kTrainThreads) void mlp_grad_kernel(
+    float const *pairs, int32_t stride, int32_t const *index, int32_t count,
+    float const *weights, float *grad, float *loss_sum, float scale) {
+  __shared__ float W2[kHidden][kHidden + 1];
+  __shared__ float sh_h1[kTrainThreads][kHidden + 1];
+  __shared__ float sh_d2[kTrainThreads][kHidden + 1];
+  __shared__ float sh_h2[kTrainThreads][kHidden + 1];
+  __shared__ float sh_d1[kTrainThreads][kHidden + 1];
+  __shared__ float sh_in[kTrainThreads][kInputs + 1];
+  __shared__ float sh_d3[kTrainThreads];
+  int const tid = threadIdx.x;
+  for (int e = tid; e < kHidden * kHidden; e += kTrainThreads)
+    W2[e / kHidden][e % kHidden] = round_bf16(weights[kW2 + e]);
+  __syncthreads();
+  int const i = blockIdx.x * kTrainThreads + tid;
+  bool const valid = i < count;
+  float u[kInputs];
+  float t = 0.0f;
+  if (valid) {
+    int64_t const row = index != nullptr ? index[i] : i;
+    for (int k = 0; k < kInputs; ++k)
+      u[k] = round_bf16(pairs[row * stride + k]);
+    t = pairs[row * stride + kInputs];
+  } else {
+    for (int k = 0; k < kInputs; ++k)
+      u[k] = 0.0f;
+  }
+  float z1[kHidden], h1[kHidden];
+  for (int h = 0; h < kHidden; ++h) {
+    float acc = round_bf16(weights[kB1 + h]);
+    for (int k = 0; k < kInputs; ++k)
+      acc += round_bf16(weights[kW1 + h * kInputs + k]) * u[k];
+    z1[h] = acc;
+    h1[h] = round_bf16(gelu(acc));
+  }
+  float r = weights[kB3];
+  float z2[kHidden];
+  for (int h = 0; h < kHidden; ++h) {
+    float acc = 0.0f;
+    for (int k = 0; k < kHidden; ++k)
+      acc += W2[h][k] * h1[k];
+    acc += weights[kB2 + h];
+    z2[h] = acc;
+    float const h2 = round_bf16(gelu(acc));
+    sh_h2[tid][h] = valid ? h2 : 0.0f;
+    r += round_bf16(weights[kW3 + h]) * h2;
+  }
+  float const err = valid ? r - t : 0.0f;
+  float const d3 = 2.0f * err * scale; // d loss / d r, scale = 1/batch
+  if (valid)
+    atomicAdd(loss_sum, err * err * scale);
+  sh_d3[tid] = d3;
+  for (int h = 0; h < kHidden; ++h) {
+    float const d2 = d3 * round_bf16(weights[kW3 + h]) * gelu_grad(z2[h]);
+    sh_d2[tid][h] = d2;
+  }
+  __syncthreads();
+  for (int k = 0; k < kHidden; ++k) {
+    float acc = 0.0f;
+    for (int h = 0; h < kHidden; ++h)
+      acc += W2[h][k] * sh_d2[tid][h];
+    sh_d1[tid][k] = acc * gelu_grad(z1[k]);
+    sh_h1[tid][k] = h1[k];
+  }
+  for (int k = 0; k < kInputs; ++k)
+    sh_in[tid][k] = u[k];
+  __syncthreads();
+  // block reductions over the batch rows held in LDS
+  for (int e = tid; e < kParams; e += kTrainThreads) {
+    float acc = 0.0f;
+    if (e < kB1) {
+      int const h = e / kInputs, k = e % kInputs;
+      for (int b = 0; b < kTrainThreads; ++b)
+        acc += sh_d1[b][h] * sh_in[b][k];
+    } else if (e < kW2) {
+      int const h = e - kB1;
+      for (int b = 0; b < kTrainThreads; ++b)
+        acc += sh_d1[b][h];
+    } else if (e < kB2) {
+      int const h = (e - kW2) / kHidden, k = (e - kW2) % kHidden;
+      for (int b = 0; b < kTrainThreads; ++b)
+        acc += sh_d2[b][h] * sh_h1[b][k];
+    } else if (e < kW3) {
+      int const h = e - kB2;
+      for (int b = 0; b < kTrainThreads; ++b)
+        acc += sh_d2[b][h];
+    } else if (e < kB3) {
+      int const h = e - kW3;
+      for (int b = 0; b < kTrainThreads; ++b)
+        acc += sh_d3[b] * sh_h2[b][h];
+    } else {
+      for (int b = 0; b < kTrainThreads; ++b)
+        acc += sh_d3[b];
+    }
+    atomicAdd(&grad[e], acc);
+  }
+}
+
+// Adam on the fp32 master weights; clears the gradient for the next step.
+__global__ void adam_kernel(float *weights, float *grad, float *m1, float *m2,
+                            float lr, float beta1, float beta2, float eps,
+                            float corr1, float corr2) {
+  int const e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= kParams)
+    return;
+  float const g = grad[e];
+  float const a = beta1 * m1[e] + (1.0f - beta1) * g;
+  float const b = beta2 * m2[e] + (1.0f - beta2) * g * g;
+  m1[e] = a;
+  m2[e] = b;
+  weights[e] -= lr * (a / corr1) / (sqrtf(b / corr2) + eps);
+  grad[e] = 0.0f;
+}
+
+__global__ void fill_kernel(float *p, int32_t count, float v) {
+  int const e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < count)
+    p[e] = v;
+}
+
+uint32_t blocks_for(int64_t work, int threads) {
+  int64_t b = (work + threads - 1) / threads;
+  return static_cast<uint32_t>(b < 1 ? 1 : (b > (1 << 20) ? (1 << 20) : b));
+}
+
+} // namespace learned
+} // namespace paliwa_hip
+
+using namespace paliwa_hip;
+using namespace paliwa_hip::learned;
+
+namespace {
+
+int hip_status(hipError_t err) {
+  if (err != hipSuccess)
+    return fail(PALIWA_ERR_HIP, std::string("HIP: ") + hipGetErrorString(err));
+  return PALIWA_OK;
+}
+
+int check_int_grid(void const *data, int32_t ndim, int64_t const *extents,
+                   int32_t const *level, int32_t const *lmin) {
+  int64_t elements = 0;
+  // the extents rules are those of the float grids
+  int rc = check_grid(data, PALIWA_FLOAT32, ndim, extents, level, nullptr,
+                      &elements);
+  if (rc != PALIWA_OK)
+    return rc;
+  if (lmin == nullptr)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "minimum_level is required");
+  for (int d = 0; d < ndim; ++d) {
+    if (lmin[d] < 0 || lmin[d] > level[d])
+      return fail(PALIWA_ERR_LEVEL, "require 0 <= minimum_level <= level");
+    int64_t const m = extents[d];
+    if (m * kTile * int64_t(sizeof(int32_t)) > 160 * 1024)
+      return fail(PALIWA_ERR_UNSUPPORTED,
+                  "learned lifting holds lines of at most 2^11 points");
+  }
+  return PALIWA_OK;
+}
+
+int learned_run(int32_t *data, int32_t ndim, int64_t const *extents,
+                int32_t const *level, int32_t const *lmin, int32_t axis,
+                float const *weights, uint32_t const *mlp_mask, int32_t update,
+                int32_t feature_kind, bool inverse, double *stats, float *pairs,
+                uint32_t *pair_count, uint32_t capacity, uint32_t threshold,
+                uint32_t salt, int32_t *overflow, hipStream_t stream) {
+  int rc = check_int_grid(data, ndim, extents, level, lmin);
+  if (rc != PALIWA_OK)
+    return rc;
+  if (axis < -1 || axis >= ndim)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "axis out of range");
+  if (feature_kind != kFeaturesRaw && feature_kind != kFeaturesDetrended)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "unknown feature kind");
+  for (int i = 0; i < ndim; ++i) {
+    int const d = inverse ? ndim - 1 - i : i;
+    if (axis >= 0 && d != axis)
+      continue;
+    SweepParams p = make_params(data, ndim, extents, level, d);
+    if (level[d] == lmin[d])
+      continue;
+    LearnedArgs a{};
+    a.data = data;
+    a.A = p.A;
+    a.B = p.B;
+    a.m = p.m;
+    a.L = p.L;
+    a.lmin = lmin[d];
+    a.inverse = inverse ? 1 : 0;
+    a.update = update;
+    a.mlp_mask = mlp_mask != nullptr ? mlp_mask[d] : 0u;
+    a.features = feature_kind;
+    a.weights = weights;
+    a.stats = stats != nullptr ? stats + size_t(d) * 2 * (PALIWA_MAX_LEVEL + 1) : nullptr;
+    a.pairs = pairs;
+    a.pair_count = pair_count;
+    a.pair_capacity = capacity;
+    a.sample_threshold = threshold;
+    a.salt = salt * 131u + uint32_t(d);
+    a.overflow = overflow;
+    if ((a.mlp_mask >> 3) != 0 && weights == nullptr)
+      return fail(PALIWA_ERR_INVALID_ARGUMENT, "weights are required");
+    int64_t const tiles = (p.A * p.B + kTile - 1) / kTile;
+    size_t const bytes = size_t(kTile) * p.m * sizeof(int32_t);
+    bool const strided = p.B > 1;
+    const void *kern = strided ? reinterpret_cast<const void *>(learned_sweep<true>)
+                               : reinterpret_cast<const void *>(learned_sweep<false>);
+    if (bytes > 65536) {
+      rc = hip_status(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          static_cast<int>(bytes)));
+      if (rc != PALIWA_OK)
+        return rc;
+    }
+    if (strided)
+      hipLaunchKernelGGL(learned_sweep<true>, dim3(uint32_t(tiles)), dim3(kThreads), bytes, stream, a);
+    else
+      hipLaunchKernelGGL(learned_sweep<false>, dim3(uint32_t(tiles)), dim3(kThreads), bytes, stream, a);
+    if ((rc = hip_status(hipGetLastError())) != PALIWA_OK)
+      return rc;
+  }
+  last_error().clear();
+  return PALIWA_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int paliwa_learned_param_count(void) { return kParams; }
+
+int paliwa_quantize_f32(float const *in, int32_t *out, int64_t count,
+                        float scale, void *stream) {
+  if (in == nullptr || out == nullptr || count < 0 || !(scale > 0.0f))
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "bad quantization arguments");
+  hipLaunchKernelGGL(quantize_kernel, dim3(blocks_for(count, 256)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), in, out, count, 1.0f / scale);
+  return hip_status(hipGetLastError());
+}
+
+int paliwa_dequantize_f32(int32_t const *in, float *out, int64_t count,
+                          float scale, void *stream) {
+  if (in == nullptr || out == nullptr || count < 0 || !(scale > 0.0f))
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "bad quantization arguments");
+  hipLaunchKernelGGL(dequantize_kernel, dim3(blocks_for(count, 256)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), in, out, count, scale);
+  return hip_status(hipGetLastError());
+}
+
+int paliwa_learned_hierarchize(int32_t *data, int32_t ndim,
+                               int64_t const *extents, int32_t const *level,
+                               int32_t const *minimum_level, int32_t axis,
+                               float const *weights, uint32_t const *mlp_mask,
+                               int32_t update, int32_t features, double *stats,
+                               int32_t *overflow, void *stream) {
+  return learned_run(data, ndim, extents, level, minimum_level, axis, weights,
+                     mlp_mask, update, features, false, stats, nullptr,
+                     nullptr, 0, 0, 0, overflow,
+                     static_cast<hipStream_t>(stream));
+}
+
+int paliwa_learned_dehierarchize(int32_t *data, int32_t ndim,
+                                 int64_t const *extents, int32_t const *level,
+                                 int32_t const *minimum_level, int32_t axis,
+                                 float const *weights,
+                                 uint32_t const *mlp_mask, int32_t update,
+                                 int32_t features, int32_t *overflow,
+                                 void *stream) {
+  return learned_run(data, ndim, extents, level, minimum_level, axis, weights,
+                     mlp_mask, update, features, true, nullptr, nullptr,
+                     nullptr, 0, 0, 0, overflow,
+                     static_cast<hipStream_t>(stream));
+}
+
+int paliwa_learned_extract_pairs(int32_t *data, int32_t ndim,
+                                 int64_t const *extents, int32_t const *level,
+                                 int32_t const *minimum_level, int32_t update,
+                                 int32_t features, float *pairs,
+                                 uint32_t *pair_count, uint32_t capacity,
+                                 uint32_t sample_threshold, uint32_t salt,
+                                 void *stream) {
+  if (pairs == nullptr || pair_count == nullptr)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "pairs and pair_count are required");
+  return learned_run(data, ndim, extents, level, minimum_level, -1, nullptr,
+                     nullptr, update, features, false, nullptr, pairs,
+                     pair_count, capacity, sample_threshold, salt, nullptr,
+                     static_cast<hipStream_t>(stream));
+}
+
+int paliwa_learned_mlp_eval(float const *inputs, float const *weights,
+                            float *out, int32_t count, void *stream) {
+  if (inputs == nullptr || weights == nullptr || out == nullptr || count < 0)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "bad arguments");
+  hipLaunchKernelGGL(mlp_eval_kernel, dim3(blocks_for(int64_t(count) * 4, kThreads)),
+                     dim3(kThreads), 0, static_cast<hipStream_t>(stream), inputs,
+                     weights, out, count);
+  return hip_status(hipGetLastError());
+}
+
+int paliwa_learned_gradient(float const *pairs, int32_t stride,
+                            int32_t const *index, int32_t count,
+                            float const *weights, float *grad,
+                            float *loss_sum, void *stream) {
+  if (pairs == nullptr || weights == nullptr || grad == nullptr ||
+      loss_sum == nullptr || count < 1 || stride < 9)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "bad arguments");
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks_for(count, kTrainThreads)),
+                     dim3(kTrainThreads), 0, static_cast<hipStream_t>(stream),
+                     pairs, stride, index, count, weights, grad, loss_sum,
+                     1.0f / float(count));
+  return hip_status(hipGetLastError());
+}
+
+int paliwa_learned_adam(float *weights, float *grad, float *m1, float *m2,
+                        float lr, float beta1, float beta2, float eps,
+                        int32_t step, void *stream) {
+  if (weights == nullptr || grad == nullptr || m1 == nullptr || m2 == nullptr ||
+      step < 1)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "bad arguments");
+  float const c1 = 1.0f - std::pow(beta1, float(step));
+  float const c2 = 1.0f - std::pow(beta2, float(step));
+  hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), weights, grad, m1, m2, lr,
+                     beta1, beta2, eps, c1, c2);
+  return hip_status(hipGetLastError());
+}
+
+int paliwa_learned_fill(float *p, int32_t count, float value, void *stream) {
+  if (p == nullptr || count < 0)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "bad arguments");
+  hipLaunchKernelGGL(fill_kernel, dim3((count + 255) / 256), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), p, count, value);
+  return hip_status(hipGetLastError());
+}
+
+} // extern "C"

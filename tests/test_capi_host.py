@@ -162,6 +162,29 @@ def test_batch_create_validation():
     assert _lib.lib.paliwa_batch_hierarchize(None, b"hat", None) == _lib.PALIWA_ERR_INVALID_ARGUMENT
 
 
+def test_combine_create_validation():
+    plan = ctypes.c_void_p(None)
+    ptrs = (ctypes.c_void_p * 2)(0x1000, 0x2000)
+    ext = _lib.i64_array([17, 16, 9, 8])
+    lv = _lib.i32_array([4, 4, 3, 3])
+    coefs = (ctypes.c_double * 2)(1.0, -1.0)
+    mx = _lib.i32_array([4, 4])
+    create = _lib.lib.paliwa_combine_create
+    bad_ext = _lib.i64_array([17, 16, 10, 8])
+    assert create(ctypes.byref(plan), 0, 2, 2, ptrs, bad_ext, lv, coefs, mx, 2) == _lib.PALIWA_ERR_SHAPE
+    assert "grid 1" in _lib.last_error()
+    low_max = _lib.i32_array([4, 3])
+    assert create(ctypes.byref(plan), 0, 2, 2, ptrs, ext, lv, coefs, low_max, 2) == _lib.PALIWA_ERR_LEVEL
+    assert create(ctypes.byref(plan), 0, 2, 2, ptrs, ext, lv, coefs, mx, 0) == _lib.PALIWA_ERR_INVALID_ARGUMENT
+    assert create(ctypes.byref(plan), 0, 9, 2, ptrs, ext, lv, coefs, mx, 2) == _lib.PALIWA_ERR_UNSUPPORTED
+    assert create(ctypes.byref(plan), 7, 2, 2, ptrs, ext, lv, coefs, mx, 2) == _lib.PALIWA_ERR_DTYPE
+    assert create(ctypes.byref(plan), 0, 2, 2, ptrs, ext, lv, None, mx, 2) == _lib.PALIWA_ERR_INVALID_ARGUMENT
+    assert plan.value is None
+    assert _lib.lib.paliwa_combine_gather(None, None, None) == _lib.PALIWA_ERR_INVALID_ARGUMENT
+    assert _lib.lib.paliwa_combine_scatter(None, None, None, None, None) == _lib.PALIWA_ERR_INVALID_ARGUMENT
+    assert _lib.lib.paliwa_combine_destroy(None) == 0
+
+
 def test_combination_levels():
     from paliwa_amd import combination_levels
 

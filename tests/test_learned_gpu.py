@@ -1,0 +1,168 @@
+"""Learned lifting kernels on the GPU: the MFMA network against the fp64
+checker (tests/learned_ref.py), the gradient kernel against fp64
+backpropagation, the exact integer round trip, the interpolet fallback against
+the numpy integer lifting, a short training run, and the full workload's gain
+over the interpolet."""
+import numpy as np
+import pytest
+import torch
+
+import learned_ref as lr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ll():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a HIP device")
+    from paliwa_amd import learned
+
+    return learned
+
+
+def random_weights(seed, scale=0.5):
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal(lr.P).astype(np.float32) * scale
+    w[lr.B3] = 0.1
+    return w
+
+
+def test_mfma_network_matches_fp64(ll):
+    rng = np.random.default_rng(31)
+    w = random_weights(31)
+    u = rng.uniform(-1, 1, size=(4000, 8)).astype(np.float32)
+    got = ll.mlp_eval(torch.from_numpy(u).cuda(), torch.from_numpy(w).cuda()).cpu().numpy()
+    want = lr.mlp(u, w)
+    diff = np.abs(got - want)
+    scale = np.abs(want).max()
+    # fp32 accumulation against fp64; a bf16 rounding of a hidden activation
+    # may fall the other way for a few points
+    assert np.quantile(diff, 0.99) <= 1e-5 * scale, np.quantile(diff, 0.99)
+    assert diff.max() <= 2e-2 * scale, diff.max()
+
+
+def test_gradient_kernel_matches_fp64(ll):
+    rng = np.random.default_rng(32)
+    w = random_weights(32, 0.3)
+    u = rng.uniform(-1, 1, size=(777, 8)).astype(np.float32)
+    t = rng.uniform(-0.3, 0.3, size=777).astype(np.float32)
+    pairs = np.concatenate([u, t[:, None], np.zeros((777, 1), np.float32)], axis=1)
+    loss, grad = ll.gradient(torch.from_numpy(pairs).cuda(), torch.from_numpy(w).cuda())
+    want_loss, want_grad = lr.mlp_grad(u.astype(np.float64), t.astype(np.float64), w)
+    assert abs(loss.item() - want_loss) <= 1e-4 * want_loss
+    g = grad.cpu().numpy()
+    assert np.linalg.norm(g - want_grad) <= 2e-3 * np.linalg.norm(want_grad)
+    # index-gathered rows give the same gradient as the rows themselves
+    idx = torch.arange(776, -1, -1, dtype=torch.int32, device="cuda")
+    loss2, grad2 = ll.gradient(torch.from_numpy(pairs).cuda(), torch.from_numpy(w).cuda(), idx)
+    assert np.allclose(grad2.cpu().numpy(), g, rtol=1e-4, atol=1e-7)
+
+
+def int_grid(rng, level, image=True, amplitude=1 << 20):
+    ext = [(1 << l) + (1 if image else 0) for l in level]
+    q = rng.integers(-amplitude, amplitude, size=ext).astype(np.int32)
+    if image:
+        for d, l in enumerate(level):
+            src = [slice(None)] * len(level)
+            dst = [slice(None)] * len(level)
+            src[d], dst[d] = 0, 1 << l
+            q[tuple(dst)] = q[tuple(src)]
+    return np.ascontiguousarray(q)
+
+
+@pytest.mark.parametrize("update", [True, False])
+def test_interpolet_fallback_matches_integer_lifting(ll, update):
+    rng = np.random.default_rng(33)
+    for level, lmin, image in (([5, 4, 6], [0, 1, 2], True), ([3, 7], [0, 0], False),
+                               ([9], [1], True)):
+        q = int_grid(rng, level, image)
+        t = torch.from_numpy(q).cuda()
+        ll.forward(t, level, lmin, None, None, update)
+        want = lr.integer_lifting(q, level, lmin, update)
+        assert np.array_equal(t.cpu().numpy(), want), (level, update)
+        ll.inverse(t, level, lmin, None, None, update)
+        assert np.array_equal(t.cpu().numpy(), q)
+
+
+@pytest.mark.parametrize("features", [0, 1])
+@pytest.mark.parametrize("update", [True, False])
+def test_learned_round_trip_is_exact(ll, update, features):
+    rng = np.random.default_rng(34)
+    # outputs of order 0.1 correction units, as a trained network gives; a
+    # network saturating the correction bound at random grows white-noise
+    # details geometrically over the levels
+    w = random_weights(34, 0.4)
+    w[lr.W3:lr.B3] *= 0.05
+    w[lr.B3] = 0.02
+    w = torch.from_numpy(w).cuda()
+    for level, lmin, image in (([6, 5, 7], [0, 1, 0], True), ([9, 4], [2, 0], False)):
+        q = int_grid(rng, level, image, amplitude=1 << 18)
+        t = torch.from_numpy(q).cuda()
+        masks = [sum(1 << l for l in range(3, L + 1)) for L in level]
+        overflow = torch.zeros(1, dtype=torch.int32, device="cuda")
+        ll.forward(t, level, lmin, w, masks, update, overflow=overflow, features=features)
+        assert overflow.item() == 0
+        fwd = t.cpu().numpy()
+        assert not np.array_equal(fwd, lr.integer_lifting(q, level, lmin, update))
+        ll.inverse(t, level, lmin, w, masks, update, overflow=overflow, features=features)
+        assert overflow.item() == 0
+        assert np.array_equal(t.cpu().numpy(), q), (level, update, features)
+
+
+def test_single_axis_calls_compose(ll):
+    """forward along each axis in turn equals the all-axes forward; inverse
+    along each axis in reverse order undoes it."""
+    rng = np.random.default_rng(35)
+    w = random_weights(35, 0.4)
+    w[lr.W3:lr.B3] *= 0.05
+    w = torch.from_numpy(w).cuda()
+    level, lmin = [6, 5, 7], [0, 1, 0]
+    q = int_grid(rng, level, True, amplitude=1 << 18)
+    masks = [0b1011000, 0b101000, 0b11110000]
+    whole = torch.from_numpy(q).cuda()
+    ll.forward(whole, level, lmin, w, masks, True)
+    parts = torch.from_numpy(q).cuda()
+    for d in range(3):
+        ll.forward(parts, level, lmin, w, masks, True, axis=d)
+    assert torch.equal(whole, parts)
+    for d in (2, 1, 0):
+        ll.inverse(parts, level, lmin, w, masks, True, axis=d)
+    assert np.array_equal(parts.cpu().numpy(), q)
+
+
+def test_extract_pairs_and_train(ll):
+    from paliwa_amd import fields
+
+    level, lmin = [6, 6, 6], [0, 0, 0]
+    grids = []
+    for seed in (100, 101):
+        f = fields.fourier_modes(level, modes=64, kmax=32, seed=seed, device="cuda",
+                                 fronts=5, front_width=0.01).to(torch.float32)
+        q, _ = ll.quantize(f)
+        grids.append(q)
+    pairs = ll.extract_pairs(grids, level, lmin, capacity=1 << 16)
+    assert pairs.shape[0] > (1 << 15)
+    assert pairs[:, :8].abs().max().item() <= 1.0
+    w0 = ll.init_weights(0)
+    loss0, _ = ll.gradient(pairs, w0)
+    w, hist = ll.train(pairs, steps=400, batch=2048, lr=2e-3, log_every=100)
+    loss1, _ = ll.gradient(pairs, w)
+    # 65^3 fields of wavenumbers up to 32 are barely resolved: little to learn
+    assert loss1.item() < 0.97 * loss0.item(), (loss0.item(), loss1.item())
+
+
+def test_learned_workload_gains_over_interpolet(ll):
+    """The 513^3 float32 workload (training on 32 fields of 129^3, seeds
+    100-131; test field seed 4): the trained predictor with the interpolet
+    fallback lowers both the detail L1 and the fraction of details above
+    1e-4 max|f| against the interpolet alone, with the plain update kept and
+    an exact integer round trip."""
+    r = ll.workload()
+    li, le = r["interpolet"], r["learned"]
+    assert r["overflow"] == 0
+    assert le["integer_round_trip_exact"] and li["integer_round_trip_exact"]
+    assert le["round_trip_rel"] <= 1e-6
+    assert r["l1_ratio_learned_vs_interpolet"] < 0.95, r["l1_ratio_learned_vs_interpolet"]
+    assert r["frac_ratio_learned_vs_interpolet"] < 0.95, r["frac_ratio_learned_vs_interpolet"]
+    assert any(r["masks"])

@@ -1,0 +1,56 @@
+"""Pins the numpy checker of the learned lifting (tests/learned_ref.py)."""
+import numpy as np
+
+import learned_ref as lr
+
+
+def test_backprop_matches_finite_differences():
+    # rounding-free network: smooth, so central differences apply
+    rng = np.random.default_rng(0)
+    w = (rng.standard_normal(lr.P) * 0.3).astype(np.float32)
+    u = rng.uniform(-1, 1, size=(40, 8))
+    t = rng.uniform(-0.5, 0.5, size=40)
+    _, g = lr.mlp_grad(u, t, w, rounding=False)
+    w64 = w.astype(np.float64)
+    for e in list(range(0, lr.P, 37)) + [lr.B3, lr.W3 + 5, lr.B1 + 3]:
+        h = 1e-6
+        wp, wm = w64.copy(), w64.copy()
+        wp[e] += h
+        wm[e] -= h
+        fp = np.mean((mlp64(u, wp) - t) ** 2)
+        fm = np.mean((mlp64(u, wm) - t) ** 2)
+        fd = (fp - fm) / (2 * h)
+        assert abs(fd - g[e]) <= 1e-6 + 1e-5 * abs(fd), (e, fd, g[e])
+
+
+def mlp64(u, w):
+    w1, b1 = w[lr.W1:lr.B1].reshape(32, 8), w[lr.B1:lr.W2]
+    w2, b2 = w[lr.W2:lr.B2].reshape(32, 32), w[lr.B2:lr.W3]
+    w3, b3 = w[lr.W3:lr.B3], w[lr.B3]
+    h1 = lr.gelu(u @ w1.T + b1)
+    h2 = lr.gelu(h1 @ w2.T + b2)
+    return h2 @ w3 + b3
+
+
+def test_bf16_rounding():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 2 ** -9, 1.0 + 3 * 2 ** -9, -2.5], dtype=np.float32)
+    # 7 stored mantissa bits: 1 + 2^-8 is a tie and goes to even (1),
+    # 1 + 2^-9 rounds down, 1 + 3*2^-9 rounds up to 1 + 2^-7
+    assert list(lr.bf16(x)) == [1.0, 1.0, 1.0, 1.0 + 2 ** -7, -2.5]
+
+
+def test_integer_lifting_round_trip_and_hat_closed_form():
+    rng = np.random.default_rng(1)
+    level, lmin = [4, 3, 5], [0, 1, 2]
+    q = rng.integers(-1000, 1000, size=[17, 8, 33])
+    q[16] = q[0]
+    q[:, :, 32] = q[:, :, 0]
+    for update in (False, True):
+        f = lr.integer_lifting(q, level, lmin, update)
+        b = lr.integer_lifting(f, level, lmin, update, inverse=True)
+        assert np.array_equal(b, q)
+    # without update, one axis: detail = x - floor((left + right + 1) / 2)
+    line = rng.integers(-50, 50, size=16)
+    f = lr.integer_lifting(line, [4], [3], update=False)
+    odd = np.arange(1, 16, 2)
+    assert np.array_equal(f[odd], line[odd] - np.floor((line[odd - 1] + line[(odd + 1) % 16] + 1) / 2))
