@@ -45,6 +45,8 @@ def parse():
                     help="wall-time budget of each CPU baseline schedule")
     ap.add_argument("--no-ct", action="store_true",
                     help="skip the combination-technique batch measurement")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-lifting workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     return ap.parse_args()
@@ -216,6 +218,7 @@ def ct_benchmark(torch, pw, fields, reps=5):
     info = batch.info()
     batch.close()
     bytes_fwd = 2 * 4 * points * 8
+    combine = combine_benchmark(torch, levels, out_batched, timed, reps)
     return {
         "workload": "1325 4-D float64 grids, |l|_1 in {13..16}, l_i >= 1, hat, "
                     "forward, periodic images",
@@ -228,6 +231,59 @@ def ct_benchmark(torch, pw, fields, reps=5):
         "batched_launches": info["launches_per_direction"],
         "per_grid_launches": 4 * len(levels),
         "max_rel_diff_batched_vs_per_grid": max_rel,
+        "combine": combine,
+    }
+
+
+def combine_benchmark(torch, levels, hierarchized, timed, reps):
+    """The step after the batched hierarchize: collect the 1325 hierarchized
+    grids onto sparse-grid storage (coefficients +1, -3, +3, -1 for
+    |l|_1 = 16, 15, 14, 13; subspaces in more than one grid)."""
+    from paliwa_amd.combine import SparseGridCombination
+
+    coef_of = {16: 1.0, 15: -3.0, 14: 3.0, 13: -1.0}
+    comb = SparseGridCombination(hierarchized, levels, [coef_of[sum(l)] for l in levels],
+                                 [13] * 4)
+    info = comb.info()
+    sparse = torch.empty(info["sparse_points"], dtype=torch.float64, device="cuda")
+    ms = timed(lambda: comb.gather(sparse), reps)
+    comb.close()
+    nbytes = (info["terms"] + info["sparse_points"]) * 8
+    return {
+        "subspaces": info["subspaces"],
+        "sparse_points": info["sparse_points"],
+        "terms": info["terms"],
+        "gather_ms": round(ms, 4),
+        "gather_GBps": round(nbytes / (ms * 1e-3) / 1e9, 1),
+        "algorithmic_bytes": nbytes,
+    }
+
+
+def learned_benchmark():
+    """The learned-lifting workload: 513^3 float32 test field (seed 4),
+    training on 32 fields of 129^3 (seeds 100-131), 2^22 pairs; the learned
+    predictor with the interpolet fallback against the interpolet alone."""
+    from paliwa_amd import learned
+
+    r = learned.workload()
+    li, le = r["interpolet"], r["learned"]
+    pts = r["field"]["points"]
+    return {
+        "workload": "learned lifting, 3-D float32 (9,9,9) = 513^3, detrended inputs, "
+                    "update kept, interpolet fallback",
+        "detail_l1": {"learned": le["detail_l1"], "interpolet": li["detail_l1"],
+                      "ratio": round(r["l1_ratio_learned_vs_interpolet"], 4)},
+        "frac_above_1e-4": {"learned": le["frac_above_1e-4"], "interpolet": li["frac_above_1e-4"],
+                            "ratio": round(r["frac_ratio_learned_vs_interpolet"], 4)},
+        "round_trip_rel": le["round_trip_rel"],
+        "integer_round_trip_exact": le["integer_round_trip_exact"],
+        "masks": r["masks"],
+        "forward_ms": le["forward_ms"],
+        "inverse_ms": le["inverse_ms"],
+        "interpolet_forward_ms": li["forward_ms"],
+        "forward_Gpoints_per_s": round(pts / (le["forward_ms"] * 1e-3) / 1e9, 2),
+        "train": {k: r["train"][k] for k in ("pairs", "steps", "batch", "loss_before",
+                                             "loss_after", "seconds")},
     }
 
 
@@ -373,6 +429,8 @@ def main():
     record["fp32"] = fp32_variant(torch, pw, original, args.steps, args.warmup)
     if not args.no_ct:
         record["ct"] = ct_benchmark(torch, pw, fields)
+    if not args.no_learned:
+        record["learned"] = learned_benchmark()
     if not args.no_cpu_baseline:
         record["cpu_baseline"] = cpu_baseline(original.cpu().numpy(), args.cpu_seconds)
     print(json.dumps(record), flush=True)
