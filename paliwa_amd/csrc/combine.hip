@@ -9,14 +9,19 @@
 // point of a counted subspace and 0 elsewhere (:292-313).
 //
 // Work layout.  A prefix p = (k_0..k_{D-2}) with K_p = its largest counted
-// k_{D-1} owns rows_p = prod 2^max(k_d - 1, 0) rows of 2^K_p positions; a
-// position y of a row is in heap order: y = 0 -> k_{D-1} = 0, y >= 1 ->
-// k_{D-1} = floor(log2 y) + 1, m = y - 2^(k-1).  Threads walk the positions of
-// all prefixes in order, so a wave past the first 64 positions of a row
-// shares one subspace (one grid list, no divergence), and the subspaces of one
-// row, which read interleaved elements of the same grid rows, run back to
-// back in one workgroup: the strided reads of the coarse subspaces hit the
-// lines the fine ones brought into L2.
+// k_{D-1} owns rows_p = prod 2^max(k_d - 1, 0) rows; row r holds the points
+// of the subspaces (p, 0..K_p) at prefix point r, 2^K_p in all, and each
+// component grid containing (p, 0) stores them in one of its rows.  A
+// workgroup covers 1024 positions of one prefix: R = 1024 / 2^K_p whole rows
+// (grouped by subspace, rows outer, m inner) or, for longer rows, 1024
+// positions of one row in heap order (y = 0 -> k = 0; y >= 1 -> k =
+// floor(log2 y) + 1, m = y - 2^(k-1)).  It loops over the grids containing
+// (p, 0), in grid order, with the grid's descriptor in scalar registers; in
+// each iteration every thread whose 4 points lie in the grid (k <= l_{D-1})
+// loads them and accumulates.  So the fold order per point is the grid order,
+// the loads of one iteration cover the block's grid rows together (the
+// strided subsets of the subspaces complete each other's cache lines), four
+// loads per thread are in flight, and the writes are contiguous runs.
 #include "common.hpp"
 
 #include <hip/hip_runtime.h>
@@ -34,36 +39,35 @@ using namespace paliwa_hip;
 constexpr int kCombineDims = PALIWA_COMBINE_MAX_DIMS;
 constexpr int kThreads = 256;
 constexpr int kPerThread = 4;
-constexpr int64_t kBlockPositions = int64_t(kThreads) * kPerThread;
-
-struct GridDesc {
-  void const *data;
-  double coefficient;
-  int64_t stride[kCombineDims]; // element stride per axis (row-major)
-  int32_t level[kCombineDims];
-};
+constexpr int kLogBlock = 10;
+constexpr int64_t kBlockPositions = int64_t(1) << kLogBlock;
+static_assert(kThreads * kPerThread == kBlockPositions, "block size");
 
 struct PrefixDesc {
-  int64_t pos_begin;  // first global position
+  int64_t rows;       // points of the prefix subspace
   int32_t first_sub;  // subspace id of (p, 0); (p, k) is first_sub + k
   int32_t K;          // largest counted k_{D-1}
   int32_t k[kCombineDims];
 };
 
-struct SubDesc {
-  int64_t offset; // first element in sparse storage
-  int32_t list_begin;
+struct BlockJob {
+  int32_t prefix;
+  int32_t local;      // block index within the prefix
+  int32_t list_begin; // first term record of the block
   int32_t list_len;
 };
 
+// A term record: one grid holding a prefix's points, D + 2 words
+// {data pointer, coefficient bits, last-axis level, mult_0..mult_{D-2}}; the
+// grid element of a point is sum_d (2 m_d + 1) mult_d over the prefix axes
+// (mult_d = stride_d << (l_d - k_d), 0 where k_d = 0) plus, along the last
+// axis, (2 m + 1) << (l - k) for k >= 1 and 0 for k = 0.
 struct CombineArgs {
-  GridDesc const *grids;
   PrefixDesc const *prefixes;
-  SubDesc const *subs;
-  int32_t const *lists;
-  int32_t const *block_prefix; // first prefix of each block, [nblocks + 1]
+  int64_t const *sub_offset; // first sparse element of each subspace
+  int64_t const *records;      // per prefix (and smallest last level), grid order
+  BlockJob const *jobs;        // [nblocks]
   int32_t const *prefix_of;    // dense prefix-level table -> prefix id or -1
-  int64_t total_positions;
   int32_t ndim;
   int32_t nprefix;
   int32_t max_level[kCombineDims];
@@ -71,11 +75,6 @@ struct CombineArgs {
 };
 
 __device__ __forceinline__ int shape_log(int k) { return k > 0 ? k - 1 : 0; }
-
-// index along one axis of a grid at level lg of the point m of subspace level k
-__device__ __forceinline__ int64_t axis_index(int k, int64_t m, int lg) {
-  return k > 0 ? ((2 * m + 1) << (lg - k)) : 0;
-}
 
 template <typename T> struct Accumulate;
 template <> struct Accumulate<double> {
@@ -94,55 +93,109 @@ template <> struct Accumulate<float> {
 template <typename T, int D>
 __global__ void __launch_bounds__(kThreads)
     combine_gather(CombineArgs a, T *__restrict__ sparse) {
-  int32_t const b = blockIdx.x;
-  int32_t const p_lo = a.block_prefix[b];
-  int32_t const p_hi = a.block_prefix[b + 1];
+  constexpr int P = D > 1 ? D - 1 : 1; // prefix axes (at least one slot)
+  BlockJob const job = a.jobs[blockIdx.x];
+  PrefixDesc const &pd = a.prefixes[job.prefix];
+  int const K = pd.K;
+  int pk[P];
+#pragma unroll
+  for (int d = 0; d < P; ++d)
+    pk[d] = d < D - 1 ? pd.k[d] : 0;
+  // this thread's points: subspace level k along the last axis, odd
+  // multipliers 2 m_d + 1 along the prefix axes, output index
+  int k[kPerThread];
+  int64_t col_odd[kPerThread];
+  int64_t o[kPerThread][P];
+  int64_t out[kPerThread];
 #pragma unroll
   for (int r = 0; r < kPerThread; ++r) {
-    int64_t const pos = int64_t(b) * kBlockPositions + r * kThreads + threadIdx.x;
-    if (pos >= a.total_positions)
-      return;
-    // the prefix holding pos: the last one with pos_begin <= pos
-    int32_t lo = p_lo, hi = p_hi;
-    while (lo < hi) {
-      int32_t const mid = (lo + hi + 1) >> 1;
-      if (a.prefixes[mid].pos_begin <= pos)
-        lo = mid;
-      else
-        hi = mid - 1;
+    int const t = r * kThreads + threadIdx.x;
+    int64_t row, m;
+    if (K <= kLogBlock) {
+      int const logR = kLogBlock - K;
+      int const u = t >> logR;
+      k[r] = u == 0 ? 0 : 32 - __clz(u);
+      int const logw = k[r] == 0 ? 0 : k[r] - 1;
+      int const within = t - (k[r] == 0 ? 0 : 1 << (logR + k[r] - 1));
+      row = (int64_t(job.local) << logR) + (within >> logw);
+      m = within & ((1 << logw) - 1);
+      if (row >= pd.rows)
+        k[r] = 64; // no point: never inside a grid
+    } else {
+      int const split = K - kLogBlock; // a row spans 2^split workgroups
+      row = int64_t(job.local) >> split;
+      int64_t const y =
+          ((int64_t(job.local) & ((int64_t(1) << split) - 1)) << kLogBlock) + t;
+      k[r] = y == 0 ? 0 : 64 - __clzll(y);
+      m = y == 0 ? 0 : y - (int64_t(1) << (k[r] - 1));
     }
-    PrefixDesc const &pd = a.prefixes[lo];
-    int64_t const q = pos - pd.pos_begin;
-    int const K = pd.K;
-    int64_t const row = q >> K;
-    int64_t const y = q & ((int64_t(1) << K) - 1);
-    int const k = y == 0 ? 0 : 64 - __clzll(y);
-    int64_t const m = y == 0 ? 0 : y - (int64_t(1) << (k - 1));
-    SubDesc const sd = a.subs[pd.first_sub + k];
-    // the row's point within the prefix levels (row-major over 2^(k_d - 1))
-    int pk[D > 1 ? D - 1 : 1];
-    int64_t mp[D > 1 ? D - 1 : 1];
+    col_odd[r] = 2 * m + 1;
     int64_t rest = row;
 #pragma unroll
     for (int d = D - 2; d >= 0; --d) {
-      pk[d] = pd.k[d];
       int const sl = shape_log(pk[d]);
-      mp[d] = rest & ((int64_t(1) << sl) - 1);
+      o[r][d] = 2 * (rest & ((int64_t(1) << sl) - 1)) + 1;
       rest >>= sl;
     }
-    T s = T(0);
-    for (int e = 0; e < sd.list_len; ++e) {
-      GridDesc const &g = a.grids[a.lists[sd.list_begin + e]];
-      int64_t idx = axis_index(k, m, g.level[D - 1]) * g.stride[D - 1];
-#pragma unroll
-      for (int d = 0; d < D - 1; ++d)
-        idx += axis_index(pk[d], mp[d], g.level[d]) * g.stride[d];
-      T const v = static_cast<T const *>(g.data)[idx];
-      s = Accumulate<T>::step(s, g.coefficient, v);
-    }
-    int64_t const width = int64_t(1) << shape_log(k);
-    sparse[sd.offset + row * width + m] = s;
+    out[r] = k[r] <= K ? a.sub_offset[pd.first_sub + k[r]] + (row << shape_log(k[r])) + m : -1;
   }
+  constexpr int W = D + 2;
+  constexpr int U = 4; // grids per step: U * kPerThread loads in flight
+  T s[kPerThread];
+#pragma unroll
+  for (int r = 0; r < kPerThread; ++r)
+    s[r] = T(0);
+  int64_t const *rec = a.records + int64_t(job.list_begin) * W;
+  int const len = job.list_len;
+  int e = 0;
+  for (; e + U <= len; e += U) {
+    T v[U][kPerThread];
+    int lg[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      int64_t const *w = rec + int64_t(e + u) * W;
+      T const *data = reinterpret_cast<T const *>(w[0]);
+      lg[u] = static_cast<int>(w[2]);
+#pragma unroll
+      for (int r = 0; r < kPerThread; ++r) {
+        if (k[r] <= lg[u]) {
+          int64_t idx = k[r] > 0 ? col_odd[r] << (lg[u] - k[r]) : 0;
+#pragma unroll
+          for (int d = 0; d < D - 1; ++d)
+            idx += o[r][d] * w[3 + d];
+          v[u][r] = data[idx];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      double const c = __longlong_as_double(rec[int64_t(e + u) * W + 1]);
+#pragma unroll
+      for (int r = 0; r < kPerThread; ++r)
+        if (k[r] <= lg[u])
+          s[r] = Accumulate<T>::step(s[r], c, v[u][r]);
+    }
+  }
+  for (; e < len; ++e) {
+    int64_t const *w = rec + int64_t(e) * W;
+    T const *data = reinterpret_cast<T const *>(w[0]);
+    double const c = __longlong_as_double(w[1]);
+    int const lgl = static_cast<int>(w[2]);
+#pragma unroll
+    for (int r = 0; r < kPerThread; ++r) {
+      if (k[r] <= lgl) {
+        int64_t idx = k[r] > 0 ? col_odd[r] << (lgl - k[r]) : 0;
+#pragma unroll
+        for (int d = 0; d < D - 1; ++d)
+          idx += o[r][d] * w[3 + d];
+        s[r] = Accumulate<T>::step(s[r], c, data[idx]);
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kPerThread; ++r)
+    if (out[r] >= 0)
+      sparse[out[r]] = s[r];
 }
 
 struct FullExtents {
@@ -183,13 +236,13 @@ __global__ void __launch_bounds__(kThreads)
   T value = T(0);
   int32_t const p = a.prefix_of[dense];
   if (p >= 0 && k <= a.prefixes[p].K) {
-    SubDesc const sd = a.subs[a.prefixes[p].first_sub + k];
+    int64_t const sub_begin = a.sub_offset[a.prefixes[p].first_sub + k];
     int64_t local = 0;
 #pragma unroll
     for (int d = 0; d < D - 1; ++d)
       local = (local << shape_log(pk[d])) + pm[d];
     local = (local << shape_log(k)) + m;
-    value = T(0) + sparse[sd.offset + local];
+    value = T(0) + sparse[sub_begin + local];
   }
   full[row_id * ext.e[D - 1] + i_last] = value;
 }
@@ -343,10 +396,8 @@ int paliwa_combine_create(paliwa_combine **plan, int32_t dtype, int32_t ndim,
   // counted subspaces in lexicographic order; prefixes (all but the last
   // level) with their largest counted last level
   std::vector<PrefixDesc> prefixes;
-  std::vector<SubDesc> subs;
-  std::vector<int32_t> lists;
-  std::map<Key, int32_t> sub_id;
-  int64_t offset = 0, positions = 0;
+  std::vector<int64_t> sub_offset;
+  int64_t offset = 0;
   for (auto const &[k, n] : count) {
     if (n < min_count)
       continue;
@@ -354,108 +405,112 @@ int paliwa_combine_create(paliwa_combine **plan, int32_t dtype, int32_t ndim,
     bool const new_prefix =
         prefixes.empty() ||
         !std::equal(prefix.begin(), prefix.end(), prefixes.back().k);
+    // counts fall with the last level: a prefix's counted last levels are
+    // 0..K without gaps
+    if (new_prefix ? k.back() != 0 : k.back() != prefixes.back().K + 1) {
+      destroy(c);
+      return fail(PALIWA_ERR_INVALID_ARGUMENT, "combine: inconsistent subspace counts");
+    }
     if (new_prefix) {
-      if (k.back() != 0) {
-        // a counted (p, k) implies a counted (p, 0): counts fall with k
-        destroy(c);
-        return fail(PALIWA_ERR_INVALID_ARGUMENT, "combine: inconsistent subspace counts");
-      }
       PrefixDesc pd{};
-      pd.first_sub = static_cast<int32_t>(subs.size());
+      pd.first_sub = static_cast<int32_t>(sub_offset.size());
       pd.K = 0;
       for (int d = 0; d + 1 < ndim; ++d)
         pd.k[d] = prefix[d];
       prefixes.push_back(pd);
     } else {
-      if (k.back() != prefixes.back().K + 1) {
-        destroy(c);
-        return fail(PALIWA_ERR_INVALID_ARGUMENT, "combine: inconsistent subspace counts");
-      }
       prefixes.back().K = k.back();
     }
     int64_t size = 1;
     for (int d = 0; d < ndim; ++d)
       size <<= (k[d] > 0 ? k[d] - 1 : 0);
-    SubDesc sd{};
-    sd.offset = offset;
-    offset += size;
-    sub_id[k] = static_cast<int32_t>(subs.size());
-    subs.push_back(sd);
+    sub_offset.push_back(offset);
     c->sub_levels.insert(c->sub_levels.end(), k.begin(), k.end());
-    c->sub_offsets.push_back(sd.offset);
+    c->sub_offsets.push_back(offset);
     c->sub_counts.push_back(n);
+    c->terms += size * n;
+    offset += size;
   }
   c->sub_offsets.push_back(offset);
-  c->nsub = static_cast<int32_t>(subs.size());
+  c->nsub = static_cast<int32_t>(sub_offset.size());
   c->sparse_points = offset;
-  // positions: each prefix's rows of 2^K positions, one prefix after another
-  for (auto &pd : prefixes) {
+  // workgroups: per prefix, 1024 positions each (whole rows when a row has
+  // at most 1024 points), each with the grids holding its points, in grid
+  // order: the grids containing (p, 0) whose last level reaches the
+  // workgroup's smallest last-axis subspace level
+  std::vector<BlockJob> jobs;
+  std::vector<int64_t> records;
+  int64_t const W = ndim + 2;
+  std::vector<int32_t> holders;
+  std::vector<int64_t> grid_stride(int64_t(ngrids) * ndim);
+  for (int32_t g = 0; g < ngrids; ++g) {
+    int64_t stride = 1;
+    for (int d = ndim - 1; d >= 0; --d) {
+      grid_stride[int64_t(g) * ndim + d] = stride;
+      stride *= extents[int64_t(g) * ndim + d];
+    }
+  }
+  for (size_t p = 0; p < prefixes.size(); ++p) {
+    PrefixDesc &pd = prefixes[p];
     int64_t rows = 1;
     for (int d = 0; d + 1 < ndim; ++d)
       rows <<= (pd.k[d] > 0 ? pd.k[d] - 1 : 0);
-    pd.pos_begin = positions;
-    positions += rows << pd.K;
-  }
-  // grid lists per subspace, in grid order
-  std::vector<std::vector<int32_t>> per_sub(subs.size());
-  for (int32_t g = 0; g < ngrids; ++g) {
-    int32_t const *l = levels + int64_t(g) * ndim;
-    Key k(ndim, 0);
-    while (true) {
-      auto it = sub_id.find(k);
-      if (it != sub_id.end())
-        per_sub[it->second].push_back(g);
-      int d = ndim - 1;
-      while (d >= 0 && k[d] == l[d]) {
-        k[d] = 0;
-        --d;
+    pd.rows = rows;
+    holders.clear();
+    for (int32_t g = 0; g < ngrids; ++g) {
+      int32_t const *l = levels + int64_t(g) * ndim;
+      bool inside = true;
+      for (int d = 0; d + 1 < ndim; ++d)
+        inside = inside && pd.k[d] <= l[d];
+      if (inside)
+        holders.push_back(g);
+    }
+    int64_t const nb = pd.K <= kLogBlock
+                           ? (rows + (int64_t(1) << (kLogBlock - pd.K)) - 1) >> (kLogBlock - pd.K)
+                           : rows << (pd.K - kLogBlock);
+    if (int64_t(jobs.size()) + nb > INT32_MAX) {
+      destroy(c);
+      return fail(PALIWA_ERR_UNSUPPORTED, "combine: sparse grid too large");
+    }
+    int32_t whole_begin = -1; // list shared by the workgroups starting a row
+    for (int64_t b = 0; b < nb; ++b) {
+      int kmin = 0;
+      if (pd.K > kLogBlock) {
+        int64_t const y0 = (b & ((int64_t(1) << (pd.K - kLogBlock)) - 1)) << kLogBlock;
+        kmin = y0 == 0 ? 0 : 64 - __builtin_clzll(static_cast<unsigned long long>(y0));
       }
-      if (d < 0)
-        break;
-      ++k[d];
+      BlockJob job{static_cast<int32_t>(p), static_cast<int32_t>(b), 0, 0};
+      if (kmin == 0 && whole_begin >= 0) {
+        job.list_begin = whole_begin;
+        job.list_len = static_cast<int32_t>(holders.size());
+      } else {
+        if (int64_t(records.size()) / W + int64_t(holders.size()) > INT32_MAX) {
+          destroy(c);
+          return fail(PALIWA_ERR_UNSUPPORTED, "combine: too many grid lists");
+        }
+        job.list_begin = static_cast<int32_t>(records.size() / W);
+        for (int32_t g : holders) {
+          int32_t const *l = levels + int64_t(g) * ndim;
+          if (l[ndim - 1] < kmin)
+            continue;
+          int64_t word = 0;
+          void const *ptr = data[g];
+          std::memcpy(&word, &ptr, sizeof(ptr));
+          records.push_back(word);
+          std::memcpy(&word, coefficients + g, sizeof(double));
+          records.push_back(word);
+          records.push_back(l[ndim - 1]);
+          for (int d = 0; d + 1 < ndim; ++d)
+            records.push_back(pd.k[d] > 0 ? grid_stride[int64_t(g) * ndim + d] << (l[d] - pd.k[d]) : 0);
+        }
+        job.list_len = static_cast<int32_t>(records.size() / W) - job.list_begin;
+        if (kmin == 0)
+          whole_begin = job.list_begin;
+      }
+      jobs.push_back(job);
     }
   }
-  for (size_t s = 0; s < subs.size(); ++s) {
-    subs[s].list_begin = static_cast<int32_t>(lists.size());
-    subs[s].list_len = static_cast<int32_t>(per_sub[s].size());
-    lists.insert(lists.end(), per_sub[s].begin(), per_sub[s].end());
-    int64_t const size = c->sub_offsets[s + 1] - c->sub_offsets[s];
-    c->terms += size * subs[s].list_len;
-  }
-  if (lists.size() > size_t(INT32_MAX)) {
-    destroy(c);
-    return fail(PALIWA_ERR_UNSUPPORTED, "combine: too many grid-subspace pairs");
-  }
-  // grids
-  std::vector<GridDesc> grids(ngrids);
-  for (int32_t g = 0; g < ngrids; ++g) {
-    GridDesc gd{};
-    gd.data = data[g];
-    gd.coefficient = coefficients[g];
-    int64_t stride = 1;
-    for (int d = ndim - 1; d >= 0; --d) {
-      gd.stride[d] = stride;
-      stride *= extents[int64_t(g) * ndim + d];
-      gd.level[d] = levels[int64_t(g) * ndim + d];
-    }
-    grids[g] = gd;
-  }
-  // blocks: first prefix of each block of positions
-  int64_t const nblocks = (positions + kBlockPositions - 1) / kBlockPositions;
-  if (nblocks > INT32_MAX) {
-    destroy(c);
-    return fail(PALIWA_ERR_UNSUPPORTED, "combine: sparse grid too large");
-  }
-  std::vector<int32_t> block_prefix(nblocks + 1);
-  {
-    size_t p = 0;
-    for (int64_t b = 0; b <= nblocks; ++b) {
-      int64_t const start = std::min(b * kBlockPositions, positions > 0 ? positions - 1 : 0);
-      while (p + 1 < prefixes.size() && prefixes[p + 1].pos_begin <= start)
-        ++p;
-      block_prefix[b] = static_cast<int32_t>(p);
-    }
-  }
+  int64_t const nblocks = static_cast<int64_t>(jobs.size());
   // dense prefix table
   std::vector<int32_t> prefix_of(dense_prefixes, -1);
   int32_t mult[kCombineDims] = {};
@@ -474,12 +529,11 @@ int paliwa_combine_create(paliwa_combine **plan, int32_t dtype, int32_t ndim,
   }
   // one device allocation holding every table
   std::vector<char> buf;
-  int64_t at_grids, at_prefixes, at_subs, at_lists, at_blocks, at_dense;
-  append_bytes(buf, grids, at_grids);
+  int64_t at_prefixes, at_subs, at_records, at_jobs, at_dense;
   append_bytes(buf, prefixes, at_prefixes);
-  append_bytes(buf, subs, at_subs);
-  append_bytes(buf, lists, at_lists);
-  append_bytes(buf, block_prefix, at_blocks);
+  append_bytes(buf, sub_offset, at_subs);
+  append_bytes(buf, records, at_records);
+  append_bytes(buf, jobs, at_jobs);
   append_bytes(buf, prefix_of, at_dense);
   c->dev_bytes = static_cast<int64_t>(buf.size());
   if (hipMalloc(&c->dev, buf.size()) != hipSuccess ||
@@ -491,13 +545,11 @@ int paliwa_combine_create(paliwa_combine **plan, int32_t dtype, int32_t ndim,
   }
   char *base = static_cast<char *>(c->dev);
   CombineArgs &a = c->args;
-  a.grids = reinterpret_cast<GridDesc const *>(base + at_grids);
   a.prefixes = reinterpret_cast<PrefixDesc const *>(base + at_prefixes);
-  a.subs = reinterpret_cast<SubDesc const *>(base + at_subs);
-  a.lists = reinterpret_cast<int32_t const *>(base + at_lists);
-  a.block_prefix = reinterpret_cast<int32_t const *>(base + at_blocks);
+  a.sub_offset = reinterpret_cast<int64_t const *>(base + at_subs);
+  a.records = reinterpret_cast<int64_t const *>(base + at_records);
+  a.jobs = reinterpret_cast<BlockJob const *>(base + at_jobs);
   a.prefix_of = reinterpret_cast<int32_t const *>(base + at_dense);
-  a.total_positions = positions;
   a.ndim = ndim;
   a.nprefix = static_cast<int32_t>(prefixes.size());
   for (int d = 0; d < ndim; ++d) {
