@@ -298,11 +298,14 @@ int paliwa_learned_mlp_eval(const float *inputs, const float *weights,
 
 /* Adds to grad the gradient of mean_i (mlp(u_i) - t_i)^2 over rows
  * index[0..count) (or 0..count) of pairs (row stride `stride` floats: 8
- * inputs then the target), and the loss to *loss_sum. */
+ * inputs then the target), and the loss to *loss_sum.  `workspace`: device
+ * floats, paliwa_learned_gradient_workspace(count) of them, for per-workgroup
+ * partial sums that are added in a fixed order: the result is deterministic. */
+int64_t paliwa_learned_gradient_workspace(int32_t count);
 int paliwa_learned_gradient(const float *pairs, int32_t stride,
                             const int32_t *index, int32_t count,
                             const float *weights, float *grad,
-                            float *loss_sum, void *stream);
+                            float *loss_sum, float *workspace, void *stream);
 
 /* One Adam step (step = 1, 2, ...) on the fp32 weights; clears grad. */
 int paliwa_learned_adam(float *weights, float *grad, float *m1, float *m2,

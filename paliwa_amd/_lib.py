@@ -106,7 +106,8 @@ SIGNATURES = {
         ctypes.c_int, [_vp, _i32, _i64p, _i32p, _i32p, _i32, _i32, _vp, _vp, ctypes.c_uint32,
                        ctypes.c_uint32, ctypes.c_uint32, _vp]),
     "paliwa_learned_mlp_eval": (ctypes.c_int, [_vp, _vp, _vp, _i32, _vp]),
-    "paliwa_learned_gradient": (ctypes.c_int, [_vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp]),
+    "paliwa_learned_gradient_workspace": (ctypes.c_int64, [_i32]),
+    "paliwa_learned_gradient": (ctypes.c_int, [_vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
     "paliwa_learned_adam": (
         ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                        ctypes.c_float, _i32, _vp]),

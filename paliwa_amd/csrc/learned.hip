@@ -457,12 +457,16 @@ __global__ __launch_bounds__(kThreads) void mlp_eval_kernel(
 //   loss = mean_i (mlp(u_i) - t_i)^2
 // Forward in fp32 on bf16-rounded weights, inputs and activations (the
 // precision of the MFMA path); backward in fp32 through the rounded values.
+// Each workgroup of 64 rows writes its partial gradient and loss to a
+// workspace row; reduce_partials then sums the rows in block order, so the
+// result does not depend on the order in which workgroups finish.
 
 constexpr int kTrainThreads = 64;
+constexpr int kPartial = kParams + 1; // gradient, then loss
 
 __global__ __launch_bounds__(kTrainThreads) void mlp_grad_kernel(
     float const *pairs, int32_t stride, int32_t const *index, int32_t count,
-    float const *weights, float *grad, float *loss_sum, float scale) {
+    float const *weights, float *partials, float scale) {
   __shared__ float W2[kHidden][kHidden + 1];
   __shared__ float sh_h1[kTrainThreads][kHidden + 1];
   __shared__ float sh_d2[kTrainThreads][kHidden + 1];
@@ -470,6 +474,7 @@ __global__ __launch_bounds__(kTrainThreads) void mlp_grad_kernel(
   __shared__ float sh_d1[kTrainThreads][kHidden + 1];
   __shared__ float sh_in[kTrainThreads][kInputs + 1];
   __shared__ float sh_d3[kTrainThreads];
+  __shared__ float sh_loss[kTrainThreads];
   int const tid = threadIdx.x;
   for (int e = tid; e < kHidden * kHidden; e += kTrainThreads)
     W2[e / kHidden][e % kHidden] = round_bf16(weights[kW2 + e]);
@@ -509,8 +514,7 @@ __global__ __launch_bounds__(kTrainThreads) void mlp_grad_kernel(
   }
   float const err = valid ? r - t : 0.0f;
   float const d3 = 2.0f * err * scale; // d loss / d r, scale = 1/batch
-  if (valid)
-    atomicAdd(loss_sum, err * err * scale);
+  sh_loss[tid] = err * err * scale;
   sh_d3[tid] = d3;
   for (int h = 0; h < kHidden; ++h) {
     float const d2 = d3 * round_bf16(weights[kW3 + h]) * gelu_grad(z2[h]);
@@ -528,9 +532,12 @@ __global__ __launch_bounds__(kTrainThreads) void mlp_grad_kernel(
     sh_in[tid][k] = u[k];
   __syncthreads();
   // block reductions over the batch rows held in LDS
-  for (int e = tid; e < kParams; e += kTrainThreads) {
+  for (int e = tid; e < kPartial; e += kTrainThreads) {
     float acc = 0.0f;
-    if (e < kB1) {
+    if (e == kParams) {
+      for (int b = 0; b < kTrainThreads; ++b)
+        acc += sh_loss[b];
+    } else if (e < kB1) {
       int const h = e / kInputs, k = e % kInputs;
       for (int b = 0; b < kTrainThreads; ++b)
         acc += sh_d1[b][h] * sh_in[b][k];
@@ -554,8 +561,22 @@ __global__ __launch_bounds__(kTrainThreads) void mlp_grad_kernel(
       for (int b = 0; b < kTrainThreads; ++b)
         acc += sh_d3[b];
     }
-    atomicAdd(&grad[e], acc);
+    partials[int64_t(blockIdx.x) * kPartial + e] = acc;
   }
+}
+
+__global__ void reduce_partials(float const *partials, int32_t blocks,
+                                float *grad, float *loss_sum) {
+  int const e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= kPartial)
+    return;
+  float acc = 0.0f;
+  for (int b = 0; b < blocks; ++b)
+    acc += partials[int64_t(b) * kPartial + e];
+  if (e < kParams)
+    grad[e] += acc;
+  else
+    *loss_sum += acc;
 }
 
 // Adam on the fp32 master weights; clears the gradient for the next step.
@@ -757,17 +778,24 @@ int paliwa_learned_mlp_eval(float const *inputs, float const *weights,
   return hip_status(hipGetLastError());
 }
 
+int64_t paliwa_learned_gradient_workspace(int32_t count) {
+  return count < 1 ? 0 : int64_t(blocks_for(count, kTrainThreads)) * kPartial;
+}
+
 int paliwa_learned_gradient(float const *pairs, int32_t stride,
                             int32_t const *index, int32_t count,
                             float const *weights, float *grad,
-                            float *loss_sum, void *stream) {
+                            float *loss_sum, float *workspace, void *stream) {
   if (pairs == nullptr || weights == nullptr || grad == nullptr ||
-      loss_sum == nullptr || count < 1 || stride < 9)
+      loss_sum == nullptr || workspace == nullptr || count < 1 || stride < 9)
     return fail(PALIWA_ERR_INVALID_ARGUMENT, "bad arguments");
-  hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks_for(count, kTrainThreads)),
-                     dim3(kTrainThreads), 0, static_cast<hipStream_t>(stream),
-                     pairs, stride, index, count, weights, grad, loss_sum,
+  auto const s = static_cast<hipStream_t>(stream);
+  int32_t const blocks = static_cast<int32_t>(blocks_for(count, kTrainThreads));
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks), dim3(kTrainThreads), 0, s,
+                     pairs, stride, index, count, weights, workspace,
                      1.0f / float(count));
+  hipLaunchKernelGGL(reduce_partials, dim3((kPartial + 255) / 256), dim3(256), 0,
+                     s, workspace, blocks, grad, loss_sum);
   return hip_status(hipGetLastError());
 }
 

@@ -113,16 +113,23 @@ def mlp_eval(inputs: torch.Tensor, weights: torch.Tensor) -> torch.Tensor:
     return out
 
 
+def _workspace(count: int, device) -> torch.Tensor:
+    n = _lib.lib.paliwa_learned_gradient_workspace(int(count))
+    return torch.empty(max(1, n), dtype=torch.float32, device=device)
+
+
 def gradient(pairs: torch.Tensor, weights: torch.Tensor, index: torch.Tensor | None = None):
-    """(loss, gradient) of mean (mlp(u) - t)^2 over the rows of pairs [n, >= 9]."""
+    """(loss, gradient) of mean (mlp(u) - t)^2 over the rows of pairs [n, >= 9]
+    (deterministic: partial sums are added in a fixed order)."""
     pairs = pairs.contiguous()
     grad = torch.zeros(PARAMS, dtype=torch.float32, device=pairs.device)
     loss = torch.zeros(1, dtype=torch.float32, device=pairs.device)
     count = pairs.shape[0] if index is None else index.numel()
+    work = _workspace(count, pairs.device)
     with torch.cuda.device(pairs.device):
         check(_lib.lib.paliwa_learned_gradient(
             _ptr(pairs), pairs.shape[1], _ptr(index), count, _ptr(weights),
-            _ptr(grad), _ptr(loss), _stream(pairs.device)))
+            _ptr(grad), _ptr(loss), _ptr(work), _stream(pairs.device)))
     return loss, grad
 
 
@@ -132,8 +139,8 @@ def extract_pairs(grids: Sequence[torch.Tensor], level, lmin, capacity: int,
     """Training pairs {8 network inputs, interpolet residual in correction
     units, clamped to the correction bound} from the interpolet transform of int32
     grids (consumed: they are
-    transformed in place), sampled uniformly to about `capacity` pairs and
-    sorted by their hash key (a deterministic order)."""
+    transformed in place), sampled uniformly to about `capacity` pairs (at
+    most `capacity`) in a deterministic order."""
     device = grids[0].device
     candidates = 0
     for g in grids:
@@ -143,21 +150,30 @@ def extract_pairs(grids: Sequence[torch.Tensor], level, lmin, capacity: int,
             lines = g.numel() // g.shape[d]
             npred = sum((1 << l) >> (level[d] - lev + 1) for lev in range(max(3, lmin[d] + 1), l + 1))
             candidates += lines * npred
-    rate = min(1.0, 1.05 * capacity / max(1, candidates))
+    rate = min(1.0, capacity / max(1, candidates))
     threshold = min(0xFFFFFFFF, int(rate * 2.0 ** 32))
-    pairs = torch.empty(capacity, 10, dtype=torch.float32, device=device)
+    # headroom: the sampled count is binomial around `capacity`
+    room = capacity + capacity // 8 + 4096
+    pairs = torch.empty(room, 10, dtype=torch.float32, device=device)
     count = torch.zeros(1, dtype=torch.int32, device=device)
     with torch.cuda.device(device):
         for i, g in enumerate(grids):
             check(_lib.lib.paliwa_learned_extract_pairs(
                 _ptr(g), g.dim(), _lib.i64_array(g.shape), _lib.i32_array(level),
                 _lib.i32_array(lmin), int(update), int(features), _ptr(pairs), _ptr(count),
-                capacity, threshold, seed * 1000 + i, _stream(device)))
-    n = min(capacity, int(count.item()))
-    pairs = pairs[:n]
-    keys = pairs[:, 9].contiguous().view(torch.int32)
-    order = torch.argsort(keys.to(torch.int64) & 0xFFFFFFFF, stable=True)
-    return pairs[order].contiguous()
+                room, threshold, seed * 1000 + i, _stream(device)))
+    got = int(count.item())
+    if got > room:
+        raise RuntimeError(f"extract_pairs: {got} sampled pairs overflow the buffer of {room}")
+    pairs = pairs[:got]
+    # a deterministic order whatever order the workgroups wrote in: the hash
+    # key, then the row's contents (equal keys and contents = equal rows)
+    words = pairs[:, :10].contiguous().view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    mult = torch.tensor([(2654435761 * (j + 1)) & 0x7FFFFFFF for j in range(9)],
+                        dtype=torch.int64, device=device)
+    content = (words[:, :9] * mult).sum(1) & 0xFFFFFFFF
+    order = torch.argsort((words[:, 9] << 32) | content, stable=True)
+    return pairs[order][:capacity].contiguous()
 
 
 def train(pairs: torch.Tensor, steps: int = 3000, batch: int = 4096, lr: float = 2e-3,
@@ -174,6 +190,7 @@ def train(pairs: torch.Tensor, steps: int = 3000, batch: int = 4096, lr: float =
     history = []
     n = pairs.shape[0]
     stream = _stream(device)
+    work = _workspace(batch, device)
     with torch.cuda.device(device):
         for step in range(1, steps + 1):
             idx = torch.randint(0, n, (batch,), generator=gen, device=device, dtype=torch.int32)
@@ -181,7 +198,7 @@ def train(pairs: torch.Tensor, steps: int = 3000, batch: int = 4096, lr: float =
             check(_lib.lib.paliwa_learned_fill(_ptr(loss), 1, ctypes.c_float(0.0), stream))
             check(_lib.lib.paliwa_learned_gradient(
                 _ptr(pairs), pairs.shape[1], _ptr(idx), batch, _ptr(w), _ptr(grad),
-                _ptr(loss), stream))
+                _ptr(loss), _ptr(work), stream))
             check(_lib.lib.paliwa_learned_adam(
                 _ptr(w), _ptr(grad), _ptr(m1), _ptr(m2), ctypes.c_float(cur_lr),
                 ctypes.c_float(0.9), ctypes.c_float(0.999), ctypes.c_float(1e-8), step,

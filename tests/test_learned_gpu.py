@@ -135,18 +135,26 @@ def test_extract_pairs_and_train(ll):
     from paliwa_amd import fields
 
     level, lmin = [6, 6, 6], [0, 0, 0]
-    grids = []
-    for seed in (100, 101):
-        f = fields.fourier_modes(level, modes=64, kmax=32, seed=seed, device="cuda",
-                                 fronts=5, front_width=0.01).to(torch.float32)
-        q, _ = ll.quantize(f)
-        grids.append(q)
-    pairs = ll.extract_pairs(grids, level, lmin, capacity=1 << 16)
+
+    def pairs_of_fields():
+        grids = []
+        for seed in (100, 101):
+            f = fields.fourier_modes(level, modes=64, kmax=32, seed=seed, device="cuda",
+                                     fronts=5, front_width=0.01).to(torch.float32)
+            q, _ = ll.quantize(f)
+            grids.append(q)
+        return ll.extract_pairs(grids, level, lmin, capacity=1 << 16)
+
+    pairs = pairs_of_fields()
     assert pairs.shape[0] > (1 << 15)
     assert pairs[:, :8].abs().max().item() <= 1.0
+    # extraction and training are deterministic
+    assert torch.equal(pairs, pairs_of_fields())
     w0 = ll.init_weights(0)
     loss0, _ = ll.gradient(pairs, w0)
     w, hist = ll.train(pairs, steps=400, batch=2048, lr=2e-3, log_every=100)
+    w_again, _ = ll.train(pairs, steps=400, batch=2048, lr=2e-3, log_every=100)
+    assert torch.equal(w, w_again)
     loss1, _ = ll.gradient(pairs, w)
     # 65^3 fields of wavenumbers up to 32 are barely resolved: little to learn
     assert loss1.item() < 0.97 * loss0.item(), (loss0.item(), loss1.item())
