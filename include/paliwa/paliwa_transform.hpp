@@ -89,6 +89,10 @@ auto extent_of(E const &e, std::size_t i) -> decltype(e[i]) {
   return e[i];
 }
 
+// number of axes of an extents object: rank() (mdspan style) or size()
+template <typename E> auto rank_of(E const &e) -> decltype(e.rank()) { return e.rank(); }
+template <typename E> auto rank_of(E const &e) -> decltype(e.size()) { return e.size(); }
+
 template <std::size_t D, typename Span>
 std::array<int64_t, D> grid_extents(Span const &grid) {
   std::array<int64_t, D> out{};

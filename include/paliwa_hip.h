@@ -44,6 +44,8 @@ extern "C" {
 #define PALIWA_ERR_DTYPE (-6)
 #define PALIWA_ERR_UNSUPPORTED (-7)      /* size limits below */
 #define PALIWA_ERR_HIP (-8)              /* a HIP runtime call failed */
+#define PALIWA_ERR_IO (-9)               /* a file could not be opened, read or
+                                            written */
 
 #define PALIWA_FLOAT64 0
 #define PALIWA_FLOAT32 1
@@ -182,6 +184,85 @@ int paliwa_batch_info(const paliwa_batch *plan, int32_t *launches,
                       int64_t *points, int64_t *descriptor_bytes);
 
 int paliwa_batch_destroy(paliwa_batch *plan);
+
+/* ---- Partial-output transform ------------------------------------------- *
+ * The reference's transform_in with an `outputs` domain
+ * (paliwa/paliwa_transform.hpp:253-344) and its dependency planner
+ * (paliwa/paliwa_domains.hpp:315-487).
+ *
+ * An inclusive periodic interval of finest-level indices on a lattice: min >
+ * max wraps around the period, {-1, -1} is empty (the reference's
+ * PeriodicBounds). */
+typedef struct paliwa_periodic_bounds {
+  int64_t min;
+  int64_t max;
+} paliwa_periodic_bounds;
+
+/* Replaces paliwa::transform_bounds (paliwa/paliwa_domains.hpp:337-444): the
+ * exact writes of every (level, pass), writes[i * nfilters + p] on the
+ * lattice of stride 2^(maximum_level - levels[i] + 1), and the required inputs
+ * of every hierarchical level h = 0..level, required[h] (level 0: the point 0;
+ * h >= 1: odd multiples of 2^(maximum_level - h)), for the output points
+ * `outputs` (on the lattice of stride 2^(maximum_level - level)).  Levels must
+ * be consecutive and monotone (PALIWA_ERR_INVALID_ARGUMENT, as the
+ * reference's std::invalid_argument); filter coefficients are not read.
+ * Host code: no device is touched.  writes and required may be NULL. */
+int paliwa_transform_bounds(paliwa_periodic_bounds outputs, int32_t level,
+                            int32_t maximum_level, const int32_t *levels,
+                            int32_t nlevels, const paliwa_filter *filters,
+                            int32_t nfilters, paliwa_periodic_bounds *writes,
+                            paliwa_periodic_bounds *required);
+
+/* Replaces paliwa::covering_periodic_bounds (paliwa/paliwa_domains.hpp:
+ * 472-487): the smallest periodic interval holding the sorted, distinct
+ * indices. */
+int paliwa_covering_periodic_bounds(const int64_t *indices, int64_t count,
+                                    int64_t period,
+                                    paliwa_periodic_bounds *cover);
+
+/* A partial-output transform along axis `dim`, stored at `level` (period
+ * P = 2^level; storage indices).  The caller's block (dense row-major,
+ * `extents`) holds the run [local_start, local_start + extents[dim]) mod P of
+ * the axis; every line of the block is transformed; the outputs are the run
+ * [output_start, output_start + output_count) mod P, inside the local run.
+ * Levels and filters as paliwa_transform_in (consecutive, monotone levels).
+ * The required inputs outside the local run are the ghosts
+ * (paliwa_partial_ghosts: their storage indices, ascending); the caller passes
+ * them as a second array shaped like the block with extents[dim] = their
+ * count.  Executing writes the output points only: ghosts and the other local
+ * points keep their values (the reference's packed-pole execution from local
+ * and remote storage, paliwa/paliwa_transform.hpp:53-210; its direct mode
+ * also leaves intermediate values in non-output points).  Outputs are
+ * bit-identical to the full transform's.  Creating a plan allocates its
+ * tables; a window of required inputs longer than 64 KiB per line is
+ * PALIWA_ERR_UNSUPPORTED. */
+typedef struct paliwa_partial paliwa_partial;
+
+int paliwa_partial_create(paliwa_partial **plan, int32_t dtype, int32_t ndim,
+                          const int64_t *extents, int32_t dim, int32_t level,
+                          const int32_t *levels, int32_t nlevels,
+                          const paliwa_filter *filters, int32_t nfilters,
+                          int64_t local_start, int64_t output_start,
+                          int64_t output_count);
+/* *count = number of ghosts; indices (may be NULL) receives them. */
+int paliwa_partial_ghosts(const paliwa_partial *plan, int64_t *count,
+                          int64_t *indices);
+int paliwa_partial_transform(const paliwa_partial *plan, void *local,
+                             const void *ghosts, void *stream);
+int paliwa_partial_destroy(paliwa_partial *plan);
+
+/* ---- Raw dumps ---------------------------------------------------------- *
+ * Replaces paliwa::dump_chunk_span_to_binary_file (paliwa/paliwa_io.hpp:
+ * 13-23): the `bytes` bytes of a device array (a grid's elements, row-major)
+ * to `filename`, headerless, native byte order -- the format the reference's
+ * postprocess.py reads with numpy.fromfile.  Synchronizes `stream` (the copy
+ * goes through a host buffer of at most 64 MiB). */
+int paliwa_dump_to_binary_file(const void *data, int64_t bytes,
+                               const char *filename, void *stream);
+/* The inverse: a file of exactly `bytes` bytes into device memory
+ * (PALIWA_ERR_SHAPE when the size differs). */
+int paliwa_load_from_binary_file(void *data, int64_t bytes,
+                                 const char *filename, void *stream);
 
 /* ---- Combination-technique combine -------------------------------------- *
  * The steps after the batched hierarchize in the reference's combination

@@ -237,4 +237,68 @@ int oracle_num_threads(void) {
 
 char const *oracle_last_error(void) { return last_error.c_str(); }
 
+// ---- partial-output planner: brute force of the stencil graph -------------
+// Restates the independent check of test/test_paliwa_plan.cpp:34-69: starting
+// from the output points, walk the passes backwards; a pass at level l with
+// parity `offset` writes the required points of its parity at that level and
+// makes their neighbours at +-2^(maximum - l) required.  No intervals and no
+// coefficients are involved.  oracle_plan_check compares a planner's periodic
+// bounds (min, max pairs; min > max wraps; -1 empty) with these sets and
+// returns 0 when every (level, pass) write set and the union of the required
+// level sets agree, else 1 + the index of the first disagreeing write set, or
+// -1 for the required set.
+static void bounds_points(int64_t lo, int64_t hi, int64_t stride, int64_t period,
+                          std::vector<char> &out) {
+  std::fill(out.begin(), out.end(), 0);
+  if (lo < 0)
+    return;
+  int64_t const last = hi + (lo > hi ? period : 0);
+  for (int64_t i = lo; i <= last; i += stride)
+    out[i % period] = 1;
+}
+
+int oracle_plan_check(int level, int maximum, int64_t start, int64_t count,
+                      int const *levels, int nlevels, int const *offsets,
+                      int nfilters, int64_t const *writes,
+                      int64_t const *required) {
+  int64_t const period = int64_t(1) << maximum;
+  int64_t const spacing = int64_t(1) << (maximum - level);
+  std::vector<char> req(period, 0);
+  for (int64_t k = 0; k < count; ++k)
+    req[((start + k) * spacing) % period] = 1;
+  std::vector<std::vector<char>> expected(size_t(nlevels) * nfilters,
+                                          std::vector<char>(period, 0));
+  for (int l = nlevels; l-- > 0;) {
+    int64_t const stride = int64_t(1) << (maximum - levels[l]);
+    for (int p = nfilters; p-- > 0;) {
+      auto &e = expected[size_t(l) * nfilters + p];
+      for (int64_t i = 0; i < period; ++i)
+        if (req[i] && i % (2 * stride) == offsets[p] * stride)
+          e[i] = 1;
+      for (int64_t i = 0; i < period; ++i)
+        if (e[i]) {
+          req[(i + period - stride) % period] = 1;
+          req[(i + stride) % period] = 1;
+        }
+    }
+  }
+  std::vector<char> got(period);
+  for (int l = 0; l < nlevels; ++l)
+    for (int p = 0; p < nfilters; ++p) {
+      size_t const k = size_t(l) * nfilters + p;
+      bounds_points(writes[2 * k], writes[2 * k + 1],
+                    int64_t(2) << (maximum - levels[l]), period, got);
+      if (got != expected[k])
+        return 1 + int(k);
+    }
+  std::vector<char> all(period, 0);
+  for (int h = 0; h <= level; ++h) {
+    int64_t const stride = h == 0 ? period : int64_t(2) << (maximum - h);
+    bounds_points(required[2 * h], required[2 * h + 1], stride, period, got);
+    for (int64_t i = 0; i < period; ++i)
+      all[i] |= got[i];
+  }
+  return all == req ? 0 : -1;
+}
+
 } // extern "C"

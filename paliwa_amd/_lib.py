@@ -22,6 +22,7 @@ PALIWA_ERR_SHAPE = -5
 PALIWA_ERR_DTYPE = -6
 PALIWA_ERR_UNSUPPORTED = -7
 PALIWA_ERR_HIP = -8
+PALIWA_ERR_IO = -9
 
 PALIWA_FLOAT64 = 0
 PALIWA_FLOAT32 = 1
@@ -41,8 +42,13 @@ class paliwa_filter(ctypes.Structure):
     ]
 
 
+class paliwa_periodic_bounds(ctypes.Structure):
+    _fields_ = [("min", ctypes.c_int64), ("max", ctypes.c_int64)]
+
+
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
+_pbp = ctypes.POINTER(paliwa_periodic_bounds)
 _fp = ctypes.POINTER(paliwa_filter)
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -86,6 +92,18 @@ SIGNATURES = {
     "paliwa_batch_set_graphs": (ctypes.c_int, [_vp, _i32]),
     "paliwa_batch_info": (ctypes.c_int, [_vp, _i32p, _i64p, _i64p]),
     "paliwa_batch_destroy": (ctypes.c_int, [_vp]),
+    "paliwa_transform_bounds": (
+        ctypes.c_int, [paliwa_periodic_bounds, _i32, _i32, _i32p, _i32, _fp, _i32, _pbp, _pbp]),
+    "paliwa_covering_periodic_bounds": (ctypes.c_int, [_i64p, ctypes.c_int64, ctypes.c_int64,
+                                                       _pbp]),
+    "paliwa_partial_create": (
+        ctypes.c_int, [ctypes.POINTER(_vp), _i32, _i32, _i64p, _i32, _i32, _i32p, _i32, _fp, _i32,
+                       ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
+    "paliwa_partial_ghosts": (ctypes.c_int, [_vp, _i64p, _i64p]),
+    "paliwa_partial_transform": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "paliwa_partial_destroy": (ctypes.c_int, [_vp]),
+    "paliwa_dump_to_binary_file": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_char_p, _vp]),
+    "paliwa_load_from_binary_file": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_char_p, _vp]),
     "paliwa_combine_create": (
         ctypes.c_int, [ctypes.POINTER(_vp), _i32, _i32, _i32, ctypes.POINTER(_vp), _i64p,
                        _i32p, ctypes.POINTER(ctypes.c_double), _i32p, _i32]),

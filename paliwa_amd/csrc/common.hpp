@@ -38,6 +38,12 @@ SweepParams make_params(void *data, int32_t ndim, int64_t const *extents,
 int sweep(int32_t dtype, SweepParams p, int32_t path, hipStream_t stream);
 void set_filters(SweepParams &p, paliwa_filter const *filters,
                  int32_t nfilters);
+// The exact dependency planner of a partial-output transform (plan.cpp).
+int plan_transform_bounds(paliwa_periodic_bounds outputs, int32_t level,
+                          int32_t maximum_level, int32_t const *levels,
+                          int32_t nlevels, paliwa_filter const *filters,
+                          int32_t nfilters, paliwa_periodic_bounds *writes,
+                          paliwa_periodic_bounds *required);
 // Level list of hierarchize_in (decreasing) or dehierarchize_in (increasing).
 void set_named_levels(SweepParams &p, int32_t level, int32_t lmin,
                       bool forward);

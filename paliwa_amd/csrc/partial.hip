@@ -1,0 +1,376 @@
+// Partial-output transform along one axis: the reference's transform_in with
+// an `outputs` domain and its packed-pole execution from local plus remote
+// storage (paliwa/paliwa_transform.hpp:253-344, 53-210).
+//
+// A plan fixes the axis (stored at `level`, period P = 2^level, storage
+// indices), the levels and passes, the local run [local_start, local_start +
+// extents[dim]) held by the caller's block and the output run inside it.  The
+// planner (plan.cpp) gives the exact writes of every pass and the required
+// inputs; the inputs outside the local run are the ghosts, which the caller
+// supplies in a second array (the block's shape with extents[dim] = number of
+// ghosts, ordered by storage index).  Every line of the block is then one
+// window of W slots: slot s holds storage index (cover_min + s) mod P, where
+// [cover_min, cover_max] is the smallest periodic interval holding every
+// required input.  A workgroup gathers the windows of TB lines into LDS (from
+// the local block or the ghosts), runs exactly the planned writes of every
+// pass in LDS (g = c0 g[lower] + c1 g + c2 g[upper], left to right, no
+// contraction: the arithmetic of the full transform, so the outputs are
+// bit-identical to it), and writes back the output points only.  Ghosts and
+// the other local points are not modified.
+#include "common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <vector>
+
+namespace {
+
+using namespace paliwa_hip;
+
+constexpr int kThreads = 256;
+constexpr int32_t kNoSource = std::numeric_limits<int32_t>::min();
+constexpr int64_t kMaxWindowBytes = 64 * 1024;
+
+struct PartialPass {
+  int32_t first;  // slot of the first write
+  int32_t count;  // writes
+  int32_t step;   // slots between writes (2 * stride)
+  int32_t stride; // slots to the neighbours
+  double c[3];
+};
+
+struct PartialArgs {
+  int32_t const *slot_source; // >= 0 local index, < 0: ghost -(g+1), kNoSource
+  PartialPass const *passes;
+  int32_t npasses;
+  int32_t W;      // window slots
+  int32_t period; // P: slot arithmetic is mod P
+  int32_t TB;     // lines per workgroup
+  int64_t lines, B, m_local, m_ghost;
+  int32_t out_slot, out_local, out_count;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    partial_lines(PartialArgs a, T *__restrict__ local, T const *__restrict__ ghosts) {
+  extern __shared__ unsigned char smem[];
+  T *t = reinterpret_cast<T *>(smem); // [W][TB]
+  int const TB = a.TB;
+  int64_t const line0 = int64_t(blockIdx.x) * TB;
+  int const n = a.W * TB;
+  for (int idx = threadIdx.x; idx < n; idx += kThreads) {
+    // along the line when lines are contiguous rows, across lines otherwise
+    int const s = a.B == 1 ? idx % a.W : idx / TB;
+    int const tb = a.B == 1 ? idx / a.W : idx % TB;
+    int64_t const line = line0 + tb;
+    T v = T(0);
+    if (line < a.lines) {
+      int64_t const ai = line / a.B, bi = line - (line / a.B) * a.B;
+      int32_t const src = a.slot_source[s];
+      if (src >= 0)
+        v = local[(ai * a.m_local + src) * a.B + bi];
+      else if (src != kNoSource)
+        v = ghosts[(ai * a.m_ghost + (-src - 1)) * a.B + bi];
+    }
+    t[s * TB + tb] = v;
+  }
+  __syncthreads();
+  for (int p = 0; p < a.npasses; ++p) {
+    PartialPass const ps = a.passes[p];
+    T const c0 = static_cast<T>(ps.c[0]);
+    T const c1 = static_cast<T>(ps.c[1]);
+    T const c2 = static_cast<T>(ps.c[2]);
+    int const work = ps.count * TB;
+    for (int idx = threadIdx.x; idx < work; idx += kThreads) {
+      int const i = idx / TB, tb = idx - (idx / TB) * TB;
+      // slots mod P (inside the window by the plan)
+      int x = ps.first + i * ps.step;
+      if (x >= a.period)
+        x -= a.period;
+      int lo = x - ps.stride;
+      if (lo < 0)
+        lo += a.period;
+      int hi = x + ps.stride;
+      if (hi >= a.period)
+        hi -= a.period;
+      T const lower = t[lo * TB + tb];
+      T const self = t[x * TB + tb];
+      T const upper = t[hi * TB + tb];
+      T const head = c0 * lower + c1 * self;
+      t[x * TB + tb] = head + c2 * upper;
+    }
+    __syncthreads();
+  }
+  int const outs = a.out_count * TB;
+  for (int idx = threadIdx.x; idx < outs; idx += kThreads) {
+    int const i = a.B == 1 ? idx % a.out_count : idx / TB;
+    int const tb = a.B == 1 ? idx / a.out_count : idx % TB;
+    int64_t const line = line0 + tb;
+    if (line >= a.lines)
+      continue;
+    int64_t const ai = line / a.B, bi = line - (line / a.B) * a.B;
+    int s = a.out_slot + i; // mod P, inside the window
+    if (s >= a.period)
+      s -= a.period;
+    int64_t j = a.out_local + i; // wraps only when the local run is the whole period
+    if (j >= a.m_local)
+      j -= a.m_local;
+    local[(ai * a.m_local + j) * a.B + bi] = t[s * TB + tb];
+  }
+}
+
+} // namespace
+
+struct paliwa_partial {
+  int32_t dtype = 0;
+  int32_t ndim = 0;
+  int32_t device = 0;
+  int32_t dim = 0;
+  int32_t blocks = 0;
+  int64_t lds_bytes = 0;
+  std::vector<int64_t> ghost_index;
+  PartialArgs args{};
+  void *dev = nullptr;
+};
+
+namespace {
+
+void destroy(paliwa_partial *p) {
+  if (p == nullptr)
+    return;
+  if (p->dev != nullptr) {
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(p->device);
+    (void)hipFree(p->dev);
+    (void)hipSetDevice(prev);
+  }
+  delete p;
+}
+
+} // namespace
+
+extern "C" {
+
+int paliwa_partial_create(paliwa_partial **plan, int32_t dtype, int32_t ndim,
+                          const int64_t *extents, int32_t dim, int32_t level,
+                          const int32_t *levels, int32_t nlevels,
+                          const paliwa_filter *filters, int32_t nfilters,
+                          int64_t local_start, int64_t output_start,
+                          int64_t output_count) {
+  if (plan == nullptr)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "plan is NULL");
+  *plan = nullptr;
+  if (dtype != PALIWA_FLOAT64 && dtype != PALIWA_FLOAT32)
+    return fail(PALIWA_ERR_DTYPE, "dtype must be PALIWA_FLOAT64 or PALIWA_FLOAT32");
+  if (ndim < 1 || ndim > PALIWA_MAX_DIMS || extents == nullptr)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "ndim in [1, 26] and extents are required");
+  if (dim < 0 || dim >= ndim)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "dim out of range");
+  if (level < 0 || level > 30)
+    return fail(PALIWA_ERR_LEVEL, "level out of range [0, 30]");
+  int64_t const P = int64_t(1) << level;
+  int64_t A = 1, B = 1;
+  for (int d = 0; d < ndim; ++d) {
+    if (extents[d] < 1)
+      return fail(PALIWA_ERR_SHAPE, "extents must be >= 1");
+    if (d < dim)
+      A *= extents[d];
+    if (d > dim)
+      B *= extents[d];
+  }
+  int64_t const m_local = extents[dim];
+  if (m_local > P)
+    return fail(PALIWA_ERR_SHAPE, "the local run is longer than the period 2^level");
+  if (local_start < 0 || local_start >= P || output_start < 0 || output_start >= P ||
+      output_count < 0 || output_count > m_local)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "local or output run outside the period");
+  int64_t const out_local = ((output_start - local_start) % P + P) % P;
+  // a local run of the whole period is circular: outputs may wrap around it
+  if (output_count > 0 && m_local < P && out_local + output_count > m_local)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "the output run is not inside the local run");
+  int rc = check_filters(filters, nfilters);
+  if (rc != PALIWA_OK)
+    return rc;
+  if (nlevels < 0 || nlevels > PALIWA_MAX_LEVELS || (nlevels > 0 && levels == nullptr))
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "levels: count in [0, 64] and array required");
+
+  // the plan in storage coordinates (maximum level = level)
+  paliwa_periodic_bounds const outputs =
+      output_count == 0 ? paliwa_periodic_bounds{-1, -1}
+                        : paliwa_periodic_bounds{output_start, (output_start + output_count - 1) % P};
+  std::vector<paliwa_periodic_bounds> writes(std::max<int64_t>(1, int64_t(nlevels) * nfilters));
+  std::vector<paliwa_periodic_bounds> required(level + 1);
+  rc = plan_transform_bounds(outputs, level, level, levels, nlevels, filters, nfilters,
+                             writes.data(), required.data());
+  if (rc != PALIWA_OK)
+    return rc;
+
+  auto *pl = new (std::nothrow) paliwa_partial;
+  if (pl == nullptr)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "out of host memory");
+  pl->dtype = dtype;
+  pl->ndim = ndim;
+  pl->dim = dim;
+  (void)hipGetDevice(&pl->device);
+  PartialArgs &a = pl->args;
+  a.lines = A * B;
+  a.B = B;
+  a.m_local = m_local;
+  a.period = static_cast<int32_t>(P);
+  if (output_count == 0 || a.lines == 0) {
+    *plan = pl;
+    return PALIWA_OK;
+  }
+
+  // required inputs: the lattice points of each level's interval (disjoint)
+  std::vector<int64_t> req;
+  for (int32_t h = 0; h <= level; ++h) {
+    paliwa_periodic_bounds const b = required[h];
+    if (b.min < 0)
+      continue;
+    int64_t const stride = h == 0 ? P : int64_t(2) << (level - h);
+    int64_t const last = b.max + (b.min > b.max ? P : 0);
+    for (int64_t x = b.min; x <= last; x += stride)
+      req.push_back(x % P);
+  }
+  std::sort(req.begin(), req.end());
+  paliwa_periodic_bounds cover{};
+  if (paliwa_covering_periodic_bounds(req.data(), static_cast<int64_t>(req.size()), P,
+                                      &cover) != PALIWA_OK) {
+    destroy(pl);
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "internal: required inputs not distinct");
+  }
+  int64_t const W = ((cover.max - cover.min) % P + P) % P + 1;
+  int64_t const elem = dtype == PALIWA_FLOAT64 ? 8 : 4;
+  if (W * elem > kMaxWindowBytes) {
+    destroy(pl);
+    return fail(PALIWA_ERR_UNSUPPORTED, "the window of required inputs exceeds 64 KiB per line");
+  }
+  auto slot_of = [&](int64_t x) { return ((x - cover.min) % P + P) % P; };
+  // slot sources; ghosts are the required inputs outside the local run
+  std::vector<int32_t> source(W, kNoSource);
+  for (int64_t x : req) {
+    int64_t const rel = ((x - local_start) % P + P) % P;
+    if (rel < m_local) {
+      source[slot_of(x)] = static_cast<int32_t>(rel);
+    } else {
+      source[slot_of(x)] = -static_cast<int32_t>(pl->ghost_index.size()) - 1;
+      pl->ghost_index.push_back(x);
+    }
+  }
+  a.m_ghost = static_cast<int64_t>(pl->ghost_index.size());
+  // the passes in execution order, in slot coordinates
+  std::vector<PartialPass> passes;
+  for (int32_t i = 0; i < nlevels; ++i) {
+    int64_t const stride = int64_t(1) << (level - levels[i]);
+    for (int32_t p = 0; p < nfilters; ++p) {
+      paliwa_periodic_bounds const w = writes[int64_t(i) * nfilters + p];
+      if (w.min < 0)
+        continue;
+      int64_t const span = ((w.max - w.min) % P + P) % P;
+      int64_t const count = span / (2 * stride) + 1;
+      int64_t const first = slot_of(w.min);
+      // every write and both its neighbours must be window slots (slot
+      // arithmetic mod P: a run of writes may pass from slot P - 1 to 0 when
+      // its lattice points skip the gap of the window)
+      for (int64_t k = 0; k < count; ++k) {
+        int64_t const x = (first + k * 2 * stride) % P;
+        if (x >= W || (x - stride + P) % P >= W || (x + stride) % P >= W) {
+          destroy(pl);
+          return fail(PALIWA_ERR_INVALID_ARGUMENT, "internal: planned pass leaves the window");
+        }
+      }
+      PartialPass ps{};
+      ps.first = static_cast<int32_t>(first);
+      ps.count = static_cast<int32_t>(count);
+      ps.step = static_cast<int32_t>(2 * stride);
+      ps.stride = static_cast<int32_t>(stride);
+      for (int j = 0; j < 3; ++j)
+        ps.c[j] = filters[p].coefficients[j];
+      passes.push_back(ps);
+    }
+  }
+  for (int64_t k = 0; k < output_count; ++k)
+    if (slot_of(output_start + k) >= W) {
+      destroy(pl);
+      return fail(PALIWA_ERR_INVALID_ARGUMENT, "internal: an output is outside the window");
+    }
+  a.npasses = static_cast<int32_t>(passes.size());
+  a.W = static_cast<int32_t>(W);
+  a.out_slot = static_cast<int32_t>(slot_of(output_start));
+  a.out_local = static_cast<int32_t>(out_local);
+  a.out_count = static_cast<int32_t>(output_count);
+  // lines per workgroup: as many windows as fit in 32 KiB, at most 64
+  int64_t tb = std::max<int64_t>(1, (32 * 1024) / (W * elem));
+  tb = std::min<int64_t>({tb, 64, a.lines});
+  a.TB = static_cast<int32_t>(tb);
+  int64_t const blocks = (a.lines + tb - 1) / tb;
+  if (blocks > INT32_MAX) {
+    destroy(pl);
+    return fail(PALIWA_ERR_UNSUPPORTED, "too many lines");
+  }
+  pl->blocks = static_cast<int32_t>(blocks);
+  pl->lds_bytes = W * tb * elem;
+  // device tables
+  size_t const src_bytes = source.size() * sizeof(int32_t);
+  size_t const pass_at = (src_bytes + 255) / 256 * 256;
+  size_t const total = pass_at + std::max<size_t>(1, passes.size()) * sizeof(PartialPass);
+  std::vector<char> buf(total, 0);
+  std::memcpy(buf.data(), source.data(), src_bytes);
+  if (!passes.empty())
+    std::memcpy(buf.data() + pass_at, passes.data(), passes.size() * sizeof(PartialPass));
+  if (hipMalloc(&pl->dev, total) != hipSuccess ||
+      hipMemcpy(pl->dev, buf.data(), total, hipMemcpyHostToDevice) != hipSuccess) {
+    if (pl->dev == nullptr)
+      (void)hipGetLastError();
+    destroy(pl);
+    return fail(PALIWA_ERR_HIP, "partial: device tables could not be allocated");
+  }
+  a.slot_source = static_cast<int32_t const *>(pl->dev);
+  a.passes = reinterpret_cast<PartialPass const *>(static_cast<char *>(pl->dev) + pass_at);
+  *plan = pl;
+  return PALIWA_OK;
+}
+
+int paliwa_partial_ghosts(const paliwa_partial *plan, int64_t *count,
+                          int64_t *indices) {
+  if (plan == nullptr || count == nullptr)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "plan and count are required");
+  *count = static_cast<int64_t>(plan->ghost_index.size());
+  if (indices != nullptr && !plan->ghost_index.empty())
+    std::memcpy(indices, plan->ghost_index.data(), plan->ghost_index.size() * sizeof(int64_t));
+  return PALIWA_OK;
+}
+
+int paliwa_partial_transform(const paliwa_partial *plan, void *local,
+                             const void *ghosts, void *stream) {
+  if (plan == nullptr)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "plan is NULL");
+  if (plan->blocks == 0)
+    return PALIWA_OK;
+  if (local == nullptr || (ghosts == nullptr && !plan->ghost_index.empty()))
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "local (and ghosts, if the plan has any) are required");
+  auto s = static_cast<hipStream_t>(stream);
+  size_t const lds = static_cast<size_t>(plan->lds_bytes);
+  if (plan->dtype == PALIWA_FLOAT64)
+    partial_lines<double><<<plan->blocks, kThreads, lds, s>>>(
+        plan->args, static_cast<double *>(local), static_cast<double const *>(ghosts));
+  else
+    partial_lines<float><<<plan->blocks, kThreads, lds, s>>>(
+        plan->args, static_cast<float *>(local), static_cast<float const *>(ghosts));
+  if (hipGetLastError() != hipSuccess)
+    return fail(PALIWA_ERR_HIP, "partial_lines launch failed");
+  return PALIWA_OK;
+}
+
+int paliwa_partial_destroy(paliwa_partial *plan) {
+  destroy(plan);
+  return PALIWA_OK;
+}
+
+} // extern "C"

@@ -90,6 +90,7 @@ _ERRORS = {
     _lib.PALIWA_ERR_DTYPE: ShapeError,
     _lib.PALIWA_ERR_UNSUPPORTED: UnsupportedError,
     _lib.PALIWA_ERR_HIP: HipError,
+    _lib.PALIWA_ERR_IO: OSError,
 }
 
 

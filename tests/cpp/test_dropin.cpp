@@ -6,6 +6,8 @@
 //   test_dropin --host-only   exception checks that fail validation before any
 //                             HIP call (runs without a GPU)
 //   test_dropin               everything (needs a GPU)
+#include <paliwa/paliwa_domains.hpp>
+#include <paliwa/paliwa_io.hpp>
 #include <paliwa/paliwa_transform.hpp>
 
 #include "../../oracle/paliwa_oracle.hpp"
@@ -14,6 +16,8 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <string>
 #include <cstring>
 #include <random>
 #include <vector>
@@ -39,7 +43,37 @@ template <typename E, typename F> bool throws(F &&f) {
   return false;
 }
 
+// test/test_paliwa_plan.cpp:199-221 through the drop-in planner header
+static void planner_checks() {
+  using Filters = std::vector<std::pair<int, std::array<double, 3>>>;
+  Filters const one{{1, {0., 1., 0.}}};
+  auto ignore = [](std::size_t, std::size_t, paliwa::PeriodicBounds) {};
+  for (auto const &levels : std::vector<std::vector<long>>{{4, 1}, {1, 4}, {1, 2, 1}})
+    CHECK(throws<std::invalid_argument>(
+        [&] { paliwa::transform_bounds({0, 0}, 4, 4, levels, one, ignore); }));
+  auto cover = [](std::vector<long> idx, long mn, long mx) {
+    auto b = paliwa::covering_periodic_bounds(idx, 64);
+    CHECK(b.min == mn && b.max == mx);
+  };
+  cover({}, -1, -1);
+  cover({12}, 12, 12);
+  cover({8, 12, 16}, 8, 16);
+  cover({0, 4, 56, 60}, 56, 4);
+  cover({0, 16, 32, 48}, 0, 48);
+  // writes are visited backwards; a hat pass writes the odd outputs
+  std::vector<std::pair<std::size_t, long>> seen;
+  auto req = paliwa::transform_bounds(
+      {4, 12}, 4, 4, std::vector<long>{4}, Filters{{1, {-.5, 1., -.5}}},
+      [&](std::size_t i, std::size_t, paliwa::PeriodicBounds w) {
+        seen.push_back({i, w.min});
+        CHECK(w.min == 5 && w.max == 11);
+      });
+  CHECK(seen.size() == 1 && req.size() == 5 && req[0].empty());
+  CHECK(req[2].min == 4 && req[2].max == 12 && req[4].min == 5 && req[4].max == 11);
+}
+
 static void host_only_checks() {
+  planner_checks();
   // Never dereferenced: every call below fails argument validation.
   paliwa::GridSpan<double, 2> g{reinterpret_cast<double *>(0x1000), {17, 16}};
   paliwa::LevelVector<2> lv{4, 4}, mn{0, 0}, mx{4, 4};
@@ -121,6 +155,20 @@ static void grid_checks(hipStream_t stream) {
   HIP_CHECK(hipStreamSynchronize(stream));
   paliwa_oracle::transform_in(want.data(), extents, level, 2, {6, 2, 4}, filters);
   CHECK(std::memcmp(got.data(), want.data(), n * sizeof(T)) == 0);
+  // raw dump (paliwa/paliwa_io.hpp): the elements, row-major, headerless
+  char const *tmp = std::getenv("TMPDIR");
+  std::string const path = std::string(tmp != nullptr ? tmp : "/tmp") + "/paliwa_dropin_dump.raw";
+  paliwa::dump_chunk_span_to_binary_file(grid, path, paliwa::HipExec{stream});
+  std::vector<T> file(n);
+  std::FILE *f = std::fopen(path.c_str(), "rb");
+  CHECK(f != nullptr);
+  CHECK(std::fread(file.data(), sizeof(T), n, f) == n);
+  CHECK(std::fgetc(f) == EOF);
+  std::fclose(f);
+  std::remove(path.c_str());
+  CHECK(std::memcmp(file.data(), want.data(), n * sizeof(T)) == 0);
+  CHECK(throws<std::runtime_error>(
+      [&] { paliwa::dump_chunk_span_to_binary_file(grid, "/nonexistent-dir/x.raw", paliwa::HipExec{stream}); }));
   HIP_CHECK(hipFree(dev));
 }
 
