@@ -40,6 +40,7 @@ namespace learned {
 
 constexpr int kThreads = 256;
 constexpr int kTile = 16;    // lines per tile
+constexpr int kLogTile = 4;
 constexpr int kHidden = 32;
 constexpr int kInputs = 8;
 // Packed fp32 parameter layout.
@@ -54,11 +55,9 @@ constexpr int kParams = kB3 + 1;               // 1377
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// fp32 -> bf16 bits, round to nearest even (finite inputs).
+// fp32 -> bf16 bits, round to nearest even (gfx950's v_cvt_pk_bf16_f32).
 __device__ __forceinline__ short to_bf16(float x) {
-  uint32_t const u = __float_as_uint(x);
-  uint32_t const r = u + 0x7fffu + ((u >> 16) & 1u);
-  return static_cast<short>(r >> 16);
+  return __builtin_bit_cast(short, static_cast<__bf16>(x));
 }
 __device__ __forceinline__ float from_bf16(short b) {
   return __uint_as_float(static_cast<uint32_t>(static_cast<uint16_t>(b)) << 16);
@@ -67,16 +66,26 @@ __device__ __forceinline__ float round_bf16(float x) {
   return from_bf16(to_bf16(x));
 }
 
-// GELU, tanh form.
-__device__ __forceinline__ float gelu(float z) {
-  float const t = tanhf(0.7978845608028654f * (z + 0.044715f * z * z * z));
-  return 0.5f * z * (1.0f + t);
+// GELU, tanh form, written as the identity 0.5 z (1 + tanh a) = z sigma(2a)
+// with a = sqrt(2/pi) (z + 0.044715 z^3): one hardware exponential and one
+// hardware reciprocal (v_exp_f32, v_rcp_f32: relative error ~1e-7, far below
+// the bf16 rounding the
+// activations get).  The transform's both directions and the training kernels
+// all use these functions, so the network the inverse evaluates is bit for
+// bit the one the forward evaluated.
+__device__ __forceinline__ float gelu_gate(float z) { // sigma(2a)
+  // exp(-2a) = exp2(z (g1 + g3 z^2)), g1 = -2 sqrt(2/pi) log2(e),
+  // g3 = 0.044715 g1: the constants folded into one v_exp_f32 argument
+  constexpr float g1 = -2.0f * 0.7978845608028654f * 1.4426950408889634f;
+  constexpr float g3 = 0.044715f * g1;
+  float const e = __builtin_amdgcn_exp2f(z * (g1 + g3 * (z * z)));
+  return __builtin_amdgcn_rcpf(1.0f + e);
 }
+__device__ __forceinline__ float gelu(float z) { return z * gelu_gate(z); }
 __device__ __forceinline__ float gelu_grad(float z) {
-  float const a = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-  float const t = tanhf(a);
+  float const g = gelu_gate(z);
   float const da = 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * z * z);
-  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * da;
+  return g + z * (2.0f * da) * g * (1.0f - g);
 }
 
 // Per-lane register fragments of the network for the MFMA evaluation.
@@ -123,33 +132,49 @@ __device__ __forceinline__ void load_net(float const *prm, Net &net) {
   net.b3 = prm[kB3];
 }
 
-// MLP on 16 points of one wavefront.  Lane l supplies inputs
-// x[4*(l/16) .. +3] of point l%16 (lanes 32..63: the bias input 1 at k = 8,
-// zeros elsewhere).  Returns the network output of point l%16 in every lane.
-__device__ __forceinline__ float mlp16(Net const &net, bf16x4 xin) {
+// MLP on NB independent groups of 16 points of one wavefront (NB chains in
+// flight hide each other's MFMA and exponential latencies).  Lane l supplies
+// inputs x[4*(l/16) .. +3] of point l%16 of each group (lanes 32..63: the
+// bias input 1 at k = 8, zeros elsewhere); out[b] is the network output of
+// point l%16 of group b, in every lane.  The arithmetic per group does not
+// depend on NB.
+template <int NB>
+__device__ __forceinline__ void mlp16(Net const &net, bf16x4 const (&xin)[NB],
+                                      float (&out)[NB]) {
   f32x4 const zero = {0.0f, 0.0f, 0.0f, 0.0f};
-  bf16x4 h1[2];
+  bf16x4 h1[NB][2];
 #pragma unroll
-  for (int blk = 0; blk < 2; ++blk) {
-    f32x4 const d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w1[blk], xin, zero, 0, 0, 0);
+  for (int blk = 0; blk < 2; ++blk)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      h1[blk][j] = to_bf16(gelu(d[j]));
-  }
-  float partial = 0.0f;
+    for (int b = 0; b < NB; ++b) {
+      f32x4 const d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w1[blk], xin[b], zero, 0, 0, 0);
 #pragma unroll
-  for (int blk = 0; blk < 2; ++blk) {
-    f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w2[blk][0], h1[0], zero, 0, 0, 0);
-    d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w2[blk][1], h1[1], d, 0, 0, 0);
+      for (int j = 0; j < 4; ++j)
+        h1[b][blk][j] = to_bf16(gelu(d[j]));
+    }
+  float partial[NB];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      partial += net.w3[blk][j] * round_bf16(gelu(d[j] + net.b2[blk][j]));
-  }
+  for (int b = 0; b < NB; ++b)
+    partial[b] = 0.0f;
+#pragma unroll
+  for (int blk = 0; blk < 2; ++blk)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w2[blk][0], h1[b][0], zero, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w2[blk][1], h1[b][1], d, 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        partial[b] += net.w3[blk][j] * round_bf16(gelu(d[j] + net.b2[blk][j]));
+    }
   // sum over the 4 lanes holding the same point (a + b == b + a exactly, so
   // every lane ends with the same bits)
-  partial += __shfl_xor(partial, 16, 64);
-  partial += __shfl_xor(partial, 32, 64);
-  return partial + net.b3;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    float v = partial[b];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    out[b] = v + net.b3;
+  }
 }
 
 // Inputs of this lane's 4 MLP slots for one predicted point: the 8 features
@@ -186,27 +211,31 @@ __device__ __forceinline__ float correction_bound(int32_t kind) {
   return kind == kFeaturesDetrended ? 1.0f : 4.0f;
 }
 
+// Features in fp32: a deterministic function of the coarse values (the
+// forward and the inverse run these same instructions on the same inputs),
+// which is all the integer lifting step needs to stay exactly invertible.
 __device__ __forceinline__ float features(int const (&c)[8], int32_t kind,
-                                          float (&u)[8], int64_t &norm) {
-  int64_t f[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k)
-    f[k] = kind == kFeaturesDetrended
-               ? 2 * int64_t(c[k]) - (int64_t(c[3]) + c[4]) -
-                     (int64_t(c[4]) - c[3]) * (2 * k - 7)
-               : int64_t(c[k]);
-  norm = 0;
+                                          float (&u)[8], float &norm) {
+  float f[8];
+  float const mid = static_cast<float>(c[3]) + static_cast<float>(c[4]);
+  float const slope = static_cast<float>(c[4]) - static_cast<float>(c[3]);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    int64_t const v = f[k] < 0 ? -f[k] : f[k];
-    norm = v > norm ? v : norm;
+    float const ck = static_cast<float>(c[k]);
+    f[k] = kind == kFeaturesDetrended
+               ? (2.0f * ck - mid) - slope * static_cast<float>(2 * k - 7)
+               : ck;
   }
-  float const s = static_cast<float>(norm);
-  float const inv = norm > 0 ? 1.0f / s : 0.0f;
+  norm = 0.0f;
 #pragma unroll
   for (int k = 0; k < 8; ++k)
-    u[k] = static_cast<float>(f[k]) * inv;
-  return kind == kFeaturesDetrended ? 0.5f * s : s; // correction unit
+    norm = fmaxf(norm, fabsf(f[k]));
+  float const inv = norm > 0.0f ? __builtin_amdgcn_rcpf(norm) : 0.0f;
+  // |u| <= 1 (the approximate reciprocal can round max|f| * inv to 1 + 2^-23)
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    u[k] = fminf(fmaxf(f[k] * inv, -1.0f), 1.0f);
+  return kind == kFeaturesDetrended ? 0.5f * norm : norm; // correction unit
 }
 
 struct LearnedArgs {
@@ -263,32 +292,43 @@ __global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
   if (any_mlp && a.weights != nullptr)
     load_net(a.weights, net);
 
-  // column / line base pointers
-  auto base_of = [&](int l) -> int32_t * {
-    int64_t const ln = line0 + l;
+  // line base offsets: lines are the (a, b) pairs of the [A][m][B] view
+  __shared__ int64_t line_base[kTile];
+  if (tid < kTile) {
+    int64_t const ln = line0 + tid;
     int64_t const aa = ln / a.B;
-    return a.data + aa * m * a.B + (ln - aa * a.B);
-  };
-  // load
-  for (int e = tid; e < kTile * m; e += kThreads) {
-    int const l = STRIDED ? e % kTile : e / m;
-    int const r = STRIDED ? e / kTile : e % m;
-    if (l < nl)
-      tile[lidx<STRIDED>(l, r, m)] = base_of(l)[int64_t(r) * a.B];
+    line_base[tid] = aa * m * a.B + (ln - aa * a.B);
   }
   __syncthreads();
+  // load: strided, the tile's columns of one row are consecutive in memory;
+  // contiguous (B == 1), the tile is nl * m consecutive elements
+  if constexpr (STRIDED) {
+    for (int e = tid; e < kTile * m; e += kThreads) {
+      int const l = e & (kTile - 1), r = e / kTile;
+      if (l < nl)
+        tile[r * kTile + l] = a.data[line_base[l] + int64_t(r) * a.B];
+    }
+  } else {
+    int32_t const *src = a.data + line_base[0];
+    for (int e = tid; e < nl * m; e += kThreads)
+      tile[e] = src[e];
+  }
+  __syncthreads();
+  int const nmask = n - 1;
 
   int const nlev = a.L - a.lmin;
   for (int step = 0; step < nlev; ++step) {
     int const lev = a.inverse ? a.lmin + 1 + step : a.L - step;
     int const s = 1 << (a.L - lev);
-    int const cnt = n / (2 * s); // odd points per line
+    int const cnt = 1 << (lev - 1); // odd points per line
     // inverse: undo the update first
     if (a.update && a.inverse) {
-      for (int p = tid; p < nl * cnt; p += kThreads) {
-        int const l = p / cnt, k = p % cnt;
+      for (int p = tid; p < kTile * cnt; p += kThreads) {
+        int const l = p & (kTile - 1), k = p >> kLogTile;
+        if (l >= nl)
+          continue;
         int const e = 2 * s * k;
-        int const lo = e == 0 ? n - s : e - s;
+        int const lo = (e - s) & nmask;
         int64_t const u = (int64_t(tile[lidx<STRIDED>(l, lo, m)]) +
                            tile[lidx<STRIDED>(l, e + s, m)] + 2) >> 2;
         tile[lidx<STRIDED>(l, e, m)] -= static_cast<int32_t>(u);
@@ -299,60 +339,74 @@ __global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
     bool const want_pairs = a.pairs != nullptr && lev >= 3;
     bool const want_stats = a.stats != nullptr && lev >= 3 && use_mlp;
     if (use_mlp || want_pairs || want_stats) {
-      // batches of 16 points per wavefront
-      int const batches = (nl * cnt + 15) / 16;
+      // batches of 16 points per wavefront: one odd point index k across the
+      // tile's 16 lines (lane & 15 = line), so the neighbour reads of a batch
+      // are 16 consecutive LDS words (strided layout) or words m apart (m odd)
       double l1_mlp = 0.0, l1_hat = 0.0;
-      for (int bt = wave; bt < batches; bt += kThreads / 64) {
-        int const p = bt * 16 + (lane & 15);
-        bool const valid = p < nl * cnt;
-        int const l = valid ? p / cnt : 0;
-        int const k = valid ? p % cnt : 0;
+      int const l = lane & (kTile - 1);
+      bool const lvalid = l < nl;
+      float const bound = correction_bound(a.features);
+      // one predicted point per lane & 15: integer prediction and update
+      auto finish = [&](int k, int const (&c)[8], float const (&u)[8],
+                        float norm, float unit, float r) {
+        int const x = s + 2 * s * k;
+        int64_t const hat = (int64_t(c[3]) + c[4] + 1) >> 1;
+        // within +-2^30 the correction converts to int32 in one instruction;
+        // a larger one would overflow the int32 detail anyway (flagged below)
+        float corr = unit * r;
+        corr = fminf(fmaxf(corr, -bound * unit), bound * unit);
+        corr = fminf(fmaxf(corr, -1073741824.0f), 1073741824.0f);
+        int64_t const pm = hat + __float2int_rn(corr);
+        int32_t const xv = tile[lidx<STRIDED>(l, x, m)];
+        if (!(lvalid && lane < 16))
+          return;
+        if (want_pairs && !a.inverse) {
+          uint32_t const key = mix(static_cast<uint32_t>(line0 + l) * 2654435761u ^
+                                   mix(static_cast<uint32_t>(x) ^ (a.salt << 8) ^
+                                       static_cast<uint32_t>(lev)));
+          if (key < a.sample_threshold && norm > 0.0f) {
+            uint32_t const slot = atomicAdd(a.pair_count, 1u);
+            if (slot < a.pair_capacity) {
+              float *pr = a.pairs + size_t(slot) * 10;
+              for (int j = 0; j < 8; ++j)
+                pr[j] = u[j];
+              // the predictor clamps its correction to +-bound units, so
+              // targets beyond that cannot be fitted: clamp them too
+              pr[8] = fminf(fmaxf(static_cast<float>(int64_t(xv) - hat) / unit, -bound), bound);
+              pr[9] = __uint_as_float(key);
+            }
+          }
+        }
+        if (want_stats) {
+          l1_mlp += fabs(double(int64_t(xv) - pm));
+          l1_hat += fabs(double(int64_t(xv) - hat));
+        }
+        int64_t const pred = use_mlp ? pm : hat;
+        int64_t const out = a.inverse ? int64_t(xv) + pred : int64_t(xv) - pred;
+        if (out > INT32_MAX || out < INT32_MIN)
+          overflow_flag = 1;
+        tile[lidx<STRIDED>(l, x, m)] = static_cast<int32_t>(out);
+      };
+      // batches of 16 points per wavefront: one odd point index k across the
+      // tile's 16 lines (lane & 15 = line), so the neighbour reads of a batch
+      // are 16 consecutive LDS words (strided layout) or words m apart (m odd)
+      for (int k = wave; k < cnt; k += kThreads / 64) {
         int const x = s + 2 * s * k;
         int c[8];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           int const off = (2 * (3 - j) + 1) * s; // 7s, 5s, 3s, s
-          c[j] = tile[lidx<STRIDED>(l, ((x - off) % n + n) % n, m)];
-          c[7 - j] = tile[lidx<STRIDED>(l, (x + off) % n, m)];
+          c[j] = tile[lidx<STRIDED>(l, (x - off) & nmask, m)];
+          c[7 - j] = tile[lidx<STRIDED>(l, (x + off) & nmask, m)];
         }
-        float u[8];
-        int64_t norm = 0;
+        float u[8], norm;
         float const unit = features(c, a.features, u, norm);
-        float const r = a.weights != nullptr ? mlp16(net, mlp_inputs(u)) : 0.0f;
-        int64_t const hat = (int64_t(c[3]) + c[4] + 1) >> 1;
-        float corr = unit * r;
-        float const bound = correction_bound(a.features);
-        corr = fminf(fmaxf(corr, -bound * unit), bound * unit);
-        int64_t const pm = hat + static_cast<int64_t>(rintf(corr));
-        int32_t const xv = tile[lidx<STRIDED>(l, x, m)];
-        if (valid && lane < 16) {
-          if (want_pairs && !a.inverse) {
-            uint32_t const key = mix(static_cast<uint32_t>(line0 + l) * 2654435761u ^
-                                     mix(static_cast<uint32_t>(x) ^ (a.salt << 8) ^
-                                         static_cast<uint32_t>(lev)));
-            if (key < a.sample_threshold && norm > 0) {
-              uint32_t const slot = atomicAdd(a.pair_count, 1u);
-              if (slot < a.pair_capacity) {
-                float *pr = a.pairs + size_t(slot) * 10;
-                for (int j = 0; j < 8; ++j)
-                  pr[j] = u[j];
-                // the predictor clamps its correction to +-bound units, so
-                // targets beyond that cannot be fitted: clamp them too
-                pr[8] = fminf(fmaxf(static_cast<float>(int64_t(xv) - hat) / unit, -bound), bound);
-                pr[9] = __uint_as_float(key);
-              }
-            }
-          }
-          if (want_stats) {
-            l1_mlp += fabs(double(int64_t(xv) - pm));
-            l1_hat += fabs(double(int64_t(xv) - hat));
-          }
-          int64_t const pred = use_mlp ? pm : hat;
-          int64_t const out = a.inverse ? int64_t(xv) + pred : int64_t(xv) - pred;
-          if (out > INT32_MAX || out < INT32_MIN)
-            overflow_flag = 1;
-          tile[lidx<STRIDED>(l, x, m)] = static_cast<int32_t>(out);
+        float r[1] = {0.0f};
+        if (a.weights != nullptr) {
+          bf16x4 const xin[1] = {mlp_inputs(u)};
+          mlp16<1>(net, xin, r);
         }
+        finish(k, c, u, norm, unit, r[0]);
       }
       if (want_stats) {
         // lanes 16..63 hold zeros
@@ -360,11 +414,13 @@ __global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
         atomicAdd(&a.stats[2 * lev + 1], l1_hat);
       }
     } else {
-      for (int p = tid; p < nl * cnt; p += kThreads) {
-        int const l = p / cnt, k = p % cnt;
+      for (int p = tid; p < kTile * cnt; p += kThreads) {
+        int const l = p & (kTile - 1), k = p >> kLogTile;
+        if (l >= nl)
+          continue;
         int const x = s + 2 * s * k;
         int const lo = x - s;
-        int const hi = x + s == n ? 0 : x + s;
+        int const hi = (x + s) & nmask;
         int64_t const hat = (int64_t(tile[lidx<STRIDED>(l, lo, m)]) +
                              tile[lidx<STRIDED>(l, hi, m)] + 1) >> 1;
         int64_t const xv = tile[lidx<STRIDED>(l, x, m)];
@@ -376,10 +432,12 @@ __global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
     }
     __syncthreads();
     if (a.update && !a.inverse) {
-      for (int p = tid; p < nl * cnt; p += kThreads) {
-        int const l = p / cnt, k = p % cnt;
+      for (int p = tid; p < kTile * cnt; p += kThreads) {
+        int const l = p & (kTile - 1), k = p >> kLogTile;
+        if (l >= nl)
+          continue;
         int const e = 2 * s * k;
-        int const lo = e == 0 ? n - s : e - s;
+        int const lo = (e - s) & nmask;
         int64_t const u = (int64_t(tile[lidx<STRIDED>(l, lo, m)]) +
                            tile[lidx<STRIDED>(l, e + s, m)] + 2) >> 2;
         int64_t const out = int64_t(tile[lidx<STRIDED>(l, e, m)]) + u;
@@ -395,11 +453,16 @@ __global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
     for (int l = tid; l < nl; l += kThreads)
       tile[lidx<STRIDED>(l, n, m)] = tile[lidx<STRIDED>(l, 0, m)];
   __syncthreads();
-  for (int e = tid; e < kTile * m; e += kThreads) {
-    int const l = STRIDED ? e % kTile : e / m;
-    int const r = STRIDED ? e / kTile : e % m;
-    if (l < nl)
-      base_of(l)[int64_t(r) * a.B] = tile[lidx<STRIDED>(l, r, m)];
+  if constexpr (STRIDED) {
+    for (int e = tid; e < kTile * m; e += kThreads) {
+      int const l = e & (kTile - 1), r = e / kTile;
+      if (l < nl)
+        a.data[line_base[l] + int64_t(r) * a.B] = tile[r * kTile + l];
+    }
+  } else {
+    int32_t *dst = a.data + line_base[0];
+    for (int e = tid; e < nl * m; e += kThreads)
+      dst[e] = tile[e];
   }
   if (tid == 0 && overflow_flag != 0 && a.overflow != nullptr)
     atomicOr(a.overflow, 1);
@@ -446,9 +509,11 @@ __global__ __launch_bounds__(kThreads) void mlp_eval_kernel(
         v = (q == 2 && j == 0) ? 1.0f : 0.0f;
       x[j] = to_bf16(v);
     }
-    float const r = mlp16(net, x);
+    bf16x4 const xs[1] = {x};
+    float r[1];
+    mlp16<1>(net, xs, r);
     if (valid && lane < 16)
-      out[p] = r;
+      out[p] = r[0];
   }
 }
 
