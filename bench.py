@@ -238,6 +238,57 @@ def ct_benchmark(torch, pw, fields, reps=5):
     }
 
 
+def per_axis_benchmark(torch, grid, level, wavelet, reps=20):
+    """Per-axis sweep times of one grid (forward and inverse), each sweep
+    timed over `reps` back-to-back launches with prebuilt C-ABI arguments
+    (HIP events on the launch stream); algorithmic GB/s = 2 * N * s / t."""
+    import ctypes
+
+    from paliwa_amd import _lib
+
+    D = len(level)
+    stream = torch.cuda.current_stream()
+    dtype = _lib.PALIWA_FLOAT64 if grid.dtype == torch.float64 else _lib.PALIWA_FLOAT32
+    nbytes = 2 * grid.numel() * grid.element_size()
+    out = {}
+    for fwd in (True, False):
+        fn = _lib.lib.paliwa_hierarchize_in if fwd else _lib.lib.paliwa_dehierarchize_in
+        for d in range(D):
+            args = (ctypes.c_void_p(grid.data_ptr()), dtype, D, _lib.i64_array(grid.shape),
+                    _lib.i32_array(level), _lib.i32_array([0] * D), _lib.i32_array(level), d,
+                    wavelet.encode(), ctypes.c_void_p(stream.cuda_stream))
+            for _ in range(3):
+                assert fn(*args) == 0, _lib.last_error()
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                fn(*args)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            out[f"{'fwd' if fwd else 'inv'}_dim{d}"] = {
+                "us": round(ms * 1e3, 2), "GBps": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    total_us = sum(v["us"] for v in out.values())
+    return {"per_axis": out, "fwd_inv_us": round(total_us, 2),
+            "GBps": round(2 * D * nbytes / (total_us * 1e-6) / 1e9, 1)}
+
+
+def other_workloads(torch, fields):
+    """The 2-D hat grid (1025^2) and the anisotropic 6-D grid (5,5,4,4,3,3)
+    with both wavelets: per-axis GB/s.  Their results are bit-identical to
+    the CPU oracle (tests/test_workloads_gpu.py)."""
+    out = {}
+    g = fields.gaussian_bump_2d((10, 10), seed=0, device="cuda")
+    out["hat_2d_1025"] = per_axis_benchmark(torch, g, [10, 10], "hat")
+    g = fields.cosine_modes_noise((5, 5, 4, 4, 3, 3), seed=3, device="cuda")
+    for wavelet in ("hat", "biorthogonal"):
+        out[f"{wavelet}_6d_33x33x17x17x9x9"] = per_axis_benchmark(
+            torch, g, [5, 5, 4, 4, 3, 3], wavelet)
+    return out
+
+
 def combine_benchmark(torch, levels, hierarchized, timed, reps):
     """The step after the batched hierarchize: collect the 1325 hierarchized
     grids onto sparse-grid storage (coefficients +1, -3, +3, -1 for
@@ -458,6 +509,7 @@ def main():
         "cpu_baseline": None,
     }
     record["fp32"] = fp32_variant(torch, pw, original, args.steps, args.warmup)
+    record["other_workloads"] = other_workloads(torch, fields)
     if not args.no_ct:
         record["ct"] = ct_benchmark(torch, pw, fields)
     if not args.no_learned:
