@@ -2,13 +2,17 @@
 // groups the axis sweeps of all grids by (axis, line period, stored extent),
 // so that one persistent fused launch covers the tiles of every grid in a
 // group, and replays the whole launch sequence of a direction as one HIP
-// graph.  Replaces the reference's per-grid loop
+// graph.  The groups of one axis touch disjoint grids, so they run
+// concurrently on a few internal streams (forked from and joined back into the
+// caller's stream around each axis): the many small groups of a set overlap
+// instead of each leaving most of the GPU idle for its launch.  Replaces the reference's per-grid loop
 //   for each grid: paliwa::hierarchize(grid, level, minimum_level,
 //                                      maximum_level, wavelet, instance)
 // (test/test_paliwa_combination_technique.cpp:164-174).
 #include "common.hpp"
 #include "lifting.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -39,6 +43,12 @@ struct paliwa_batch {
   void *dev = nullptr;       // one allocation holding every descriptor table
   size_t dev_bytes = 0;
 
+  // internal streams (default priority) and fork / join events
+  std::vector<hipStream_t> lanes;
+  std::vector<hipEvent_t> joins;
+  hipEvent_t fork = nullptr;
+  std::vector<int32_t> lane_of; // per group: its internal stream
+
   bool use_graphs = true;
   std::mutex mutex;
   std::map<std::pair<int, int>, hipGraphExec_t> graphs; // (wavelet, forward)
@@ -49,34 +59,76 @@ struct paliwa_batch {
 
 namespace {
 
+int launch_group(paliwa_batch &b, paliwa_batch::Group const &grp,
+                 paliwa_filter const *filters, int nfilters, bool forward,
+                 hipStream_t stream) {
+  if (grp.fused) {
+    SweepParams p{};
+    p.m = grp.m;
+    p.L = grp.L;
+    set_filters(p, filters, nfilters);
+    set_named_levels(p, grp.L, b.lmin[grp.dim], forward);
+    classify(p);
+    hipError_t const err = launch_fused_batched(
+        b.dtype, p, grp.strided, ManyGrids{grp.grids_dev, grp.tile_grid_dev},
+        grp.tiles, stream);
+    if (err != hipSuccess)
+      return fail(PALIWA_ERR_HIP, std::string("HIP: ") + hipGetErrorString(err));
+    return PALIWA_OK;
+  }
+  for (int32_t g : grp.members) {
+    SweepParams p = make_params(b.data[g], b.ndim, b.extents_of(g),
+                                b.level_of(g), grp.dim);
+    set_filters(p, filters, nfilters);
+    set_named_levels(p, grp.L, b.lmin[grp.dim], forward);
+    int const rc = sweep(b.dtype, p, PALIWA_PATH_AUTO, stream);
+    if (rc != PALIWA_OK)
+      return rc;
+  }
+  return PALIWA_OK;
+}
+
 int launch_sequence(paliwa_batch &b, WaveletEntry const &w, bool forward,
                     hipStream_t stream) {
   paliwa_filter const *filters = forward ? w.forward : w.reconstruct;
   int const nfilters = forward ? w.nforward : w.nreconstruct;
-  for (auto const &grp : b.groups) {
-    if (grp.fused) {
-      SweepParams p{};
-      p.m = grp.m;
-      p.L = grp.L;
-      set_filters(p, filters, nfilters);
-      set_named_levels(p, grp.L, b.lmin[grp.dim], forward);
-      classify(p);
-      hipError_t const err = launch_fused_batched(
-          b.dtype, p, grp.strided, ManyGrids{grp.grids_dev, grp.tile_grid_dev},
-          grp.tiles, stream);
-      if (err != hipSuccess)
-        return fail(PALIWA_ERR_HIP, std::string("HIP: ") + hipGetErrorString(err));
-    } else {
-      for (int32_t g : grp.members) {
-        SweepParams p = make_params(b.data[g], b.ndim, b.extents_of(g),
-                                    b.level_of(g), grp.dim);
-        set_filters(p, filters, nfilters);
-        set_named_levels(p, grp.L, b.lmin[grp.dim], forward);
-        int const rc = sweep(b.dtype, p, PALIWA_PATH_AUTO, stream);
+  auto hip = [](hipError_t e) {
+    return e == hipSuccess ? PALIWA_OK
+                           : fail(PALIWA_ERR_HIP, std::string("HIP: ") + hipGetErrorString(e));
+  };
+  size_t i = 0;
+  while (i < b.groups.size()) {
+    // the groups of one axis
+    size_t j = i;
+    while (j < b.groups.size() && b.groups[j].dim == b.groups[i].dim)
+      ++j;
+    if (j - i == 1 || b.lanes.empty()) {
+      for (size_t k = i; k < j; ++k) {
+        int const rc = launch_group(b, b.groups[k], filters, nfilters, forward, stream);
         if (rc != PALIWA_OK)
           return rc;
       }
+    } else {
+      int rc = hip(hipEventRecord(b.fork, stream));
+      std::vector<bool> used(b.lanes.size(), false);
+      for (size_t k = i; k < j && rc == PALIWA_OK; ++k)
+        used[b.lane_of[k]] = true;
+      for (size_t t = 0; t < b.lanes.size() && rc == PALIWA_OK; ++t)
+        if (used[t])
+          rc = hip(hipStreamWaitEvent(b.lanes[t], b.fork, 0));
+      for (size_t k = i; k < j && rc == PALIWA_OK; ++k)
+        rc = launch_group(b, b.groups[k], filters, nfilters, forward,
+                          b.lanes[b.lane_of[k]]);
+      for (size_t t = 0; t < b.lanes.size() && rc == PALIWA_OK; ++t)
+        if (used[t]) {
+          rc = hip(hipEventRecord(b.joins[t], b.lanes[t]));
+          if (rc == PALIWA_OK)
+            rc = hip(hipStreamWaitEvent(stream, b.joins[t], 0));
+        }
+      if (rc != PALIWA_OK)
+        return rc;
     }
+    i = j;
   }
   return PALIWA_OK;
 }
@@ -147,6 +199,14 @@ int execute(paliwa_batch *b, char const *wavelet_name, bool forward,
 void release(paliwa_batch *b) {
   for (auto &kv : b->graphs)
     (void)hipGraphExecDestroy(kv.second);
+  for (hipStream_t s : b->lanes)
+    if (s != nullptr)
+      (void)hipStreamDestroy(s);
+  for (hipEvent_t e : b->joins)
+    if (e != nullptr)
+      (void)hipEventDestroy(e);
+  if (b->fork != nullptr)
+    (void)hipEventDestroy(b->fork);
   if (b->dev != nullptr)
     (void)hipFree(b->dev);
   delete b;
@@ -263,6 +323,55 @@ int paliwa_batch_create(paliwa_batch **plan, int32_t dtype, int32_t ndim,
       auto *base = static_cast<unsigned char *>(b->dev);
       b->groups[i].grids_dev = reinterpret_cast<BatchGrid *>(base + grid_off[i]);
       b->groups[i].tile_grid_dev = reinterpret_cast<int32_t *>(base + tile_off[i]);
+    }
+  }
+  // Lanes for the groups of each axis: largest group first onto the least
+  // loaded lane (work estimated by the points swept).
+  constexpr int kLanes = 4; // 3: 1.57 ms, 8: 1.61 ms, 4: 1.43 ms on the 1325-grid set
+  b->lane_of.assign(b->groups.size(), 0);
+  bool concurrent = false;
+  for (size_t i = 0; i < b->groups.size();) {
+    size_t j = i;
+    while (j < b->groups.size() && b->groups[j].dim == b->groups[i].dim)
+      ++j;
+    if (j - i > 1) {
+      concurrent = true;
+      std::vector<std::pair<int64_t, size_t>> work;
+      for (size_t k = i; k < j; ++k) {
+        int64_t pts = 0;
+        for (int32_t g : b->groups[k].members) {
+          int64_t e = 1;
+          for (int32_t d = 0; d < ndim; ++d)
+            e *= b->extents_of(g)[d];
+          pts += e;
+        }
+        work.push_back({pts, k});
+      }
+      std::sort(work.begin(), work.end(), [](auto const &x, auto const &y) {
+        return x.first != y.first ? x.first > y.first : x.second < y.second;
+      });
+      int64_t load[kLanes] = {};
+      for (auto const &[pts, k] : work) {
+        int best = 0;
+        for (int t = 1; t < kLanes; ++t)
+          if (load[t] < load[best])
+            best = t;
+        load[best] += pts;
+        b->lane_of[k] = best;
+      }
+    }
+    i = j;
+  }
+  if (concurrent) {
+    b->lanes.assign(kLanes, nullptr);
+    b->joins.assign(kLanes, nullptr);
+    bool ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+    for (int t = 0; t < kLanes && ok; ++t)
+      ok = hipStreamCreateWithFlags(&b->lanes[t], hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&b->joins[t], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      release(b);
+      return fail(PALIWA_ERR_HIP, "could not create the plan's streams");
     }
   }
   *plan = b;
