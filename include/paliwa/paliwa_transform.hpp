@@ -5,6 +5,7 @@
 //   hierarchize / dehierarchize            paliwa/paliwa_transform.hpp:401-431
 //   hierarchize_in<Dim> / dehierarchize_in  paliwa/paliwa_transform.hpp:346-399
 //   transform_in<Dim> (complete grid)       paliwa/paliwa_transform.hpp:215-251
+//   transform_in<Dim> (partial outputs)     paliwa/paliwa_transform.hpp:253-344
 // The work runs in the HIP kernels of libpaliwa_hip.so through the C ABI of
 // paliwa_hip.h.  Link with -lpaliwa_hip (paliwa_amd/lib).
 //
@@ -139,6 +140,59 @@ bool transform_in(
       static_cast<int32_t>(D), ext.data(), lv.data(), mx.data(),
       static_cast<int32_t>(Dim), ls.data(), static_cast<int32_t>(ls.size()),
       fs.data(), static_cast<int32_t>(fs.size()), instance.stream));
+  return true;
+}
+
+// The output points along the transformed axis for the partial-output form:
+// `count` consecutive lattice points (at the grid's level) starting at the
+// finest-level index `start` (the reference's `outputs` domain along Dim;
+// along the other axes every line of the grid is transformed).
+struct AxisRun {
+  long start = 0;
+  long count = 0;
+};
+
+// paliwa/paliwa_transform.hpp:331-344 (outputs held in a single allocation):
+// the grid holds the whole axis Dim (2^level[Dim] points, no periodic image);
+// only the output points are written, bit-identical to the full transform
+// (the other points keep their values).  The levels must be consecutive and
+// monotone (std::invalid_argument otherwise, as the reference's planner).
+// The call plans on the host, allocates the plan's small device tables,
+// runs, and synchronizes the stream before releasing them; the C ABI's
+// paliwa_partial_* plan reuses one plan, also with ghost values of a rank's
+// block.
+template <std::size_t Dim, typename Chunk, typename LevelRange,
+          typename ExecSpace = HipExec, std::size_t D>
+bool transform_in(
+    Chunk grid, LevelVector<D> const &level, LevelVector<D> const &maximum_level,
+    LevelRange const &levels,
+    std::vector<std::pair<int, std::array<double, 3>>> const &filters,
+    AxisRun const &outputs, ExecSpace instance = ExecSpace()) {
+  static_assert(Dim < D, "axis out of range");
+  auto const ext = detail::grid_extents<D>(grid);
+  long const spacing_log2 = maximum_level[Dim] - level[Dim];
+  if (spacing_log2 < 0 || outputs.count < 0 ||
+      (outputs.start & ((1L << spacing_log2) - 1)) != 0)
+    throw std::invalid_argument("outputs are not on the grid's lattice");
+  std::vector<int32_t> ls;
+  for (auto l : levels)
+    ls.push_back(static_cast<int32_t>(l));
+  std::vector<paliwa_filter> fs;
+  for (auto const &[offset, c] : filters)
+    fs.push_back(paliwa_filter{offset, 0, {c[0], c[1], c[2]}});
+  paliwa_partial *plan = nullptr;
+  detail::throw_on_status(paliwa_partial_create(
+      &plan, detail::dtype_code<detail::value_type_of<Chunk>>(),
+      static_cast<int32_t>(D), ext.data(), static_cast<int32_t>(Dim),
+      static_cast<int32_t>(level[Dim]), ls.data(), static_cast<int32_t>(ls.size()),
+      fs.data(), static_cast<int32_t>(fs.size()), 0, outputs.start >> spacing_log2,
+      outputs.count));
+  int const rc = paliwa_partial_transform(plan, grid.data_handle(), nullptr, instance.stream);
+  hipError_t const sync = hipStreamSynchronize(instance.stream);
+  paliwa_partial_destroy(plan);
+  detail::throw_on_status(rc);
+  if (sync != hipSuccess)
+    throw std::runtime_error(hipGetErrorString(sync));
   return true;
 }
 

@@ -155,6 +155,44 @@ static void grid_checks(hipStream_t stream) {
   HIP_CHECK(hipStreamSynchronize(stream));
   paliwa_oracle::transform_in(want.data(), extents, level, 2, {6, 2, 4}, filters);
   CHECK(std::memcmp(got.data(), want.data(), n * sizeof(T)) == 0);
+  // partial outputs (paliwa/paliwa_transform.hpp:331-344) on a grid without
+  // images along the axis: only the requested run changes, bit-exactly
+  {
+    constexpr std::size_t D2 = 2;
+    paliwa::LevelVector<D2> lv2{6, 4}, mx2{7, 4};
+    std::array<int64_t, D2> e2{64, 16};
+    std::size_t const n2 = 64 * 16;
+    std::vector<T> h2(n2);
+    for (auto &x : h2)
+      x = static_cast<T>(u(rng));
+    T *d2 = nullptr;
+    HIP_CHECK(hipMalloc(&d2, n2 * sizeof(T)));
+    paliwa::GridSpan<T, D2> g2{d2, e2};
+    std::vector<std::pair<int, std::array<double, 3>>> bio{{1, {-0.5, 1.0, -0.5}},
+                                                           {0, {0.25, 1.0, 0.25}}};
+    std::vector<long> lv_list{6, 5, 4, 3, 2, 1};
+    for (auto [start, count] : std::vector<std::pair<long, long>>{{120, 8}, {34, 5}, {0, 64}}) {
+      HIP_CHECK(hipMemcpyAsync(d2, h2.data(), n2 * sizeof(T), hipMemcpyHostToDevice, stream));
+      paliwa::transform_in<0>(g2, lv2, mx2, lv_list, bio, paliwa::AxisRun{start, count},
+                              paliwa::HipExec{stream});
+      std::vector<T> got2(n2), want2 = h2;
+      HIP_CHECK(hipMemcpyAsync(got2.data(), d2, n2 * sizeof(T), hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+      paliwa_oracle::transform_in(want2.data(), {64, 16}, {6, 4}, 0, {6, 5, 4, 3, 2, 1}, bio);
+      for (long i = 0; i < 64; ++i) {
+        bool const out = ((i - start / 2) % 64 + 64) % 64 < count;
+        for (long j = 0; j < 16; ++j) {
+          T const w = out ? want2[i * 16 + j] : h2[i * 16 + j];
+          CHECK(std::memcmp(&got2[i * 16 + j], &w, sizeof(T)) == 0);
+        }
+      }
+    }
+    CHECK(throws<std::invalid_argument>([&] {
+      paliwa::transform_in<0>(g2, lv2, mx2, std::vector<long>{6, 4}, bio,
+                              paliwa::AxisRun{0, 4}, paliwa::HipExec{stream});
+    }));
+    HIP_CHECK(hipFree(d2));
+  }
   // raw dump (paliwa/paliwa_io.hpp): the elements, row-major, headerless
   char const *tmp = std::getenv("TMPDIR");
   std::string const path = std::string(tmp != nullptr ? tmp : "/tmp") + "/paliwa_dropin_dump.raw";
