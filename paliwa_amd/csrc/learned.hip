@@ -267,6 +267,17 @@ __device__ __forceinline__ uint32_t mix(uint32_t h) {
   return h;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (bid % 8); giving each
+// XCD one contiguous range of tiles puts neighbouring tiles, which share the
+// cache lines at their edges (a strided tile row is 64 bytes), in one L2.
+__device__ __forceinline__ int64_t xcd_tile(uint32_t bid, uint32_t nb) {
+  constexpr uint32_t kXcds = 8;
+  uint32_t const xcd = bid % kXcds, local = bid / kXcds;
+  uint32_t const q = nb / kXcds, r = nb % kXcds;
+  return xcd < r ? int64_t(xcd) * (q + 1) + local
+                 : int64_t(r) * (q + 1) + int64_t(xcd - r) * q + local;
+}
+
 template <bool STRIDED>
 __device__ __forceinline__ int lidx(int line, int r, int m) {
   return STRIDED ? r * kTile + line : line * m + r;
@@ -282,7 +293,7 @@ __global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
   int const m = a.m;
   int const n = 1 << a.L;
   int64_t const lines = a.A * a.B;
-  int64_t const line0 = int64_t(blockIdx.x) * kTile;
+  int64_t const line0 = xcd_tile(blockIdx.x, gridDim.x) * kTile;
   int const nl = static_cast<int>(lines - line0 < kTile ? lines - line0 : kTile);
   if (tid == 0)
     overflow_flag = 0;
@@ -343,13 +354,59 @@ __global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
       // tile's 16 lines (lane & 15 = line), so the neighbour reads of a batch
       // are 16 consecutive LDS words (strided layout) or words m apart (m odd)
       double l1_mlp = 0.0, l1_hat = 0.0;
-      int const l = lane & (kTile - 1);
-      bool const lvalid = l < nl;
       float const bound = correction_bound(a.features);
-      // one predicted point per lane & 15: integer prediction and update
-      auto finish = [&](int k, int const (&c)[8], float const (&u)[8],
-                        float norm, float unit, float r) {
-        int const x = s + 2 * s * k;
+      int const l = lane & (kTile - 1);
+      int const q = lane >> 4;
+      // A wavefront takes 4 odd point indices k at a time across the tile's
+      // 16 lines: lane (q, l) owns point (line l, k = k0 + q), gathers its
+      // neighbours and computes its features (64 distinct points per
+      // instruction; the neighbour reads of 16 lanes are 16 consecutive LDS
+      // words in the strided layout, words m apart (m odd) otherwise).  The
+      // network then runs as 4 MFMA groups of 16 points: group g takes the
+      // bf16 features of lanes 16g..16g+15 by shuffles, and lane (q, l)
+      // keeps the output of group q.
+      for (int k0 = 4 * wave; k0 < cnt; k0 += 4 * (kThreads / 64)) {
+        int const k = k0 + q;
+        bool const valid = l < nl && k < cnt;
+        int const kk = k < cnt ? k : k0;
+        int const x = s + 2 * s * kk;
+        int c[8];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          int const off = (2 * (3 - j) + 1) * s; // 7s, 5s, 3s, s
+          c[j] = tile[lidx<STRIDED>(l, (x - off) & nmask, m)];
+          c[7 - j] = tile[lidx<STRIDED>(l, (x + off) & nmask, m)];
+        }
+        float u[8], norm;
+        float const unit = features(c, a.features, u, norm);
+        float r = 0.0f;
+        if (a.weights != nullptr) {
+          uint32_t word[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            word[j] = static_cast<uint16_t>(to_bf16(u[2 * j])) |
+                      (uint32_t(static_cast<uint16_t>(to_bf16(u[2 * j + 1]))) << 16);
+#pragma unroll
+          for (int g = 0; g < 4; ++g) {
+            int const src = 16 * g + l;
+            uint32_t w[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+              w[j] = static_cast<uint32_t>(__shfl(static_cast<int>(word[j]), src, 64));
+            // lanes q = 0, 1: inputs 4q..4q+3; q = 2: the bias input 1; q = 3: 0
+            uint32_t const lo = q == 0 ? w[0] : q == 1 ? w[2] : q == 2 ? 0x3f80u : 0u;
+            uint32_t const hi = q == 0 ? w[1] : q == 1 ? w[3] : 0u;
+            bf16x4 xin;
+            xin[0] = static_cast<short>(lo & 0xffffu);
+            xin[1] = static_cast<short>(lo >> 16);
+            xin[2] = static_cast<short>(hi & 0xffffu);
+            xin[3] = static_cast<short>(hi >> 16);
+            bf16x4 const xs[1] = {xin};
+            float out[1];
+            mlp16<1>(net, xs, out);
+            r = g == q ? out[0] : r;
+          }
+        }
         int64_t const hat = (int64_t(c[3]) + c[4] + 1) >> 1;
         // within +-2^30 the correction converts to int32 in one instruction;
         // a larger one would overflow the int32 detail anyway (flagged below)
@@ -358,8 +415,8 @@ __global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
         corr = fminf(fmaxf(corr, -1073741824.0f), 1073741824.0f);
         int64_t const pm = hat + __float2int_rn(corr);
         int32_t const xv = tile[lidx<STRIDED>(l, x, m)];
-        if (!(lvalid && lane < 16))
-          return;
+        if (!valid)
+          continue;
         if (want_pairs && !a.inverse) {
           uint32_t const key = mix(static_cast<uint32_t>(line0 + l) * 2654435761u ^
                                    mix(static_cast<uint32_t>(x) ^ (a.salt << 8) ^
@@ -386,30 +443,8 @@ __global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
         if (out > INT32_MAX || out < INT32_MIN)
           overflow_flag = 1;
         tile[lidx<STRIDED>(l, x, m)] = static_cast<int32_t>(out);
-      };
-      // batches of 16 points per wavefront: one odd point index k across the
-      // tile's 16 lines (lane & 15 = line), so the neighbour reads of a batch
-      // are 16 consecutive LDS words (strided layout) or words m apart (m odd)
-      for (int k = wave; k < cnt; k += kThreads / 64) {
-        int const x = s + 2 * s * k;
-        int c[8];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          int const off = (2 * (3 - j) + 1) * s; // 7s, 5s, 3s, s
-          c[j] = tile[lidx<STRIDED>(l, (x - off) & nmask, m)];
-          c[7 - j] = tile[lidx<STRIDED>(l, (x + off) & nmask, m)];
-        }
-        float u[8], norm;
-        float const unit = features(c, a.features, u, norm);
-        float r[1] = {0.0f};
-        if (a.weights != nullptr) {
-          bf16x4 const xin[1] = {mlp_inputs(u)};
-          mlp16<1>(net, xin, r);
-        }
-        finish(k, c, u, norm, unit, r[0]);
       }
       if (want_stats) {
-        // lanes 16..63 hold zeros
         atomicAdd(&a.stats[2 * lev + 0], l1_mlp);
         atomicAdd(&a.stats[2 * lev + 1], l1_hat);
       }
