@@ -1,5 +1,6 @@
-// The fused axis-sweep kernel (device code, included by fused_f64.hip and
-// fused_f32.hip, one translation unit per dtype so they compile in parallel).
+// The fused axis-sweep kernel (device code, included by fused_f64.hip,
+// fused_f32.hip and fused_i32.hip, one translation unit per value type so they
+// compile in parallel).
 //
 // A workgroup of 256 threads owns a tile of TB = 256/G whole lines.  Each line
 // of period n = K*G is split into G chunks of K consecutive points, one chunk
@@ -35,6 +36,8 @@
 #pragma once
 
 #include "lifting.hpp"
+
+#include <type_traits>
 
 namespace paliwa_hip {
 namespace fused {
@@ -83,6 +86,25 @@ __device__ __forceinline__ int64_t xcd_tile(uint32_t bid, uint32_t nb) {
 template <typename T>
 __device__ __forceinline__ T lift(T c0, T lo, T c1, T x, T c2, T hi) {
   return c0 * lo + c1 * x + c2 * hi;
+}
+
+// Sticky per-workgroup flag of the int32 sweeps: some result left the int32
+// range.  Cleared at the start of sweep_tiles, reported at its end.
+__device__ __forceinline__ int *int_overflow_flag() {
+  __shared__ int flag;
+  return &flag;
+}
+
+// int32 lifting step (the integer interpolet predict and CDF 5/3 update):
+// the coefficient slots carry (sign, bias, shift) and the result is
+// x + sign * floor((lo + hi + bias) / 2^shift), evaluated in 64 bits.
+__device__ __forceinline__ int32_t lift(int32_t sign, int32_t lo, int32_t bias,
+                                        int32_t x, int32_t shift, int32_t hi) {
+  int64_t const q = (int64_t(lo) + hi + bias) >> shift;
+  int64_t const out = sign > 0 ? int64_t(x) + q : int64_t(x) - q;
+  if (out != int64_t(static_cast<int32_t>(out)))
+    *int_overflow_flag() = 1;
+  return static_cast<int32_t>(out);
 }
 
 // One pass at storage stride S < K on this lane's chunk.
@@ -421,6 +443,12 @@ __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
       run_generic<T, K, G>(v, gbase, gi, p);
   };
 
+  constexpr bool kInt = std::is_integral_v<T>;
+  if constexpr (kInt) {
+    if (tid == 0)
+      *int_overflow_flag() = 0;
+    __syncthreads();
+  }
   int64_t t = xcd_tile(blockIdx.x, nwg);
   if constexpr (STRIDED && (G == 1 || kStridedDirect)) {
     // Lanes load their own chunks straight between HBM and registers: lane
@@ -498,6 +526,11 @@ __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
       store_tile<T, K, G, STRIDED>(p, cur, lds, magic_m, LP);
       cur = nxt;
     }
+  }
+  if constexpr (kInt) {
+    __syncthreads();
+    if (tid == 0 && *int_overflow_flag() != 0 && p.overflow != nullptr)
+      atomicOr(p.overflow, 1);
   }
 }
 

@@ -782,6 +782,39 @@ int learned_run(int32_t *data, int32_t ndim, int64_t const *extents,
     a.overflow = overflow;
     if ((a.mlp_mask >> 3) != 0 && weights == nullptr)
       return fail(PALIWA_ERR_INVALID_ARGUMENT, "weights are required");
+    // levels lmin+1 .. L that would run the network
+    uint32_t const in_range = ((2u << p.L) - 1u) & ~((2u << lmin[d]) - 1u) & ~7u;
+    if ((a.mlp_mask & in_range) == 0 && pairs == nullptr) {
+      // interpolet only: the fused register ladder on int32, whose passes
+      // compute x + sign * floor((lo + hi + bias) / 2^shift)
+      int const nlev = p.L - lmin[d];
+      p.nlevels = nlev;
+      for (int i = 0; i < nlev; ++i)
+        p.shift[i] = static_cast<int8_t>(inverse ? nlev - 1 - i : i);
+      struct Pass {
+        int8_t odd;
+        double sign, bias, shift;
+      };
+      Pass const predict{1, inverse ? 1.0 : -1.0, 1.0, 1.0};
+      Pass const upd{0, inverse ? -1.0 : 1.0, 2.0, 2.0};
+      Pass passes[2] = {predict, upd};
+      p.npass = update ? 2 : 1;
+      if (update && inverse) {
+        passes[0] = upd;
+        passes[1] = predict;
+      }
+      for (int f = 0; f < p.npass; ++f) {
+        p.offset[f] = passes[f].odd;
+        p.coef[f][0] = passes[f].sign;
+        p.coef[f][1] = passes[f].bias;
+        p.coef[f][2] = passes[f].shift;
+      }
+      p.overflow = overflow;
+      classify(p);
+      if ((rc = hip_status(launch_fused_i32(p, stream))) != PALIWA_OK)
+        return rc;
+      continue;
+    }
     int64_t const tiles = (p.A * p.B + kTile - 1) / kTile;
     size_t const bytes = size_t(kTile) * p.m * sizeof(int32_t);
     bool const strided = p.B > 1;

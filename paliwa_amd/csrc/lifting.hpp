@@ -31,6 +31,8 @@ struct SweepParams {
   int32_t schedule; // kSchedule*
   int32_t pattern;  // kPat* (ladder schedules only)
   int32_t lo, hi;   // log2 stride range (ladder schedules only)
+  // int32 sweeps only: set to 1 when a result left the int32 range (or null)
+  int32_t *overflow;
 };
 
 enum Schedule : int32_t {
@@ -97,6 +99,10 @@ int lds_line_max_period(int dtype);
 hipError_t launch_fused(int dtype, SweepParams const &p, hipStream_t stream);
 hipError_t launch_fused_f64(SweepParams const &p, hipStream_t stream);
 hipError_t launch_fused_f32(SweepParams const &p, hipStream_t stream);
+// int32 ladders (the learned transform's interpolet sweeps): coefficient slots
+// carry {sign, bias, shift} of x + sign * floor((lo + hi + bias) / 2^shift);
+// line periods 2^1 .. 2^11, ladder schedules only.
+hipError_t launch_fused_i32(SweepParams const &p, hipStream_t stream);
 
 // Points per lane (chunk length K) of the fused kernel for a line of period
 // 2^L; the kernel's tile holds 256*K / 2^L lines.

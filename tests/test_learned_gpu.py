@@ -131,6 +131,60 @@ def test_single_axis_calls_compose(ll):
     assert np.array_equal(parts.cpu().numpy(), q)
 
 
+def test_network_free_axes_compose_with_learned_axis(ll):
+    """Axes without network levels run the fused int32 ladder, the axis with
+    network levels the learned sweep: the forward equals the numpy integer
+    lifting of axis 0, the learned sweep of axis 1, the numpy integer lifting
+    of axis 2 (one axis at a time: the other axes' level ranges empty), and the
+    inverse undoes it exactly."""
+    rng = np.random.default_rng(36)
+    w = random_weights(36, 0.4)
+    w[lr.W3:lr.B3] *= 0.05
+    w = torch.from_numpy(w).cuda()
+    level, lmin = [7, 6, 8], [1, 0, 2]
+    q = int_grid(rng, level, True, amplitude=1 << 18)
+    masks = [0, 0b1101000, 0]
+    got = torch.from_numpy(q).cuda()
+    overflow = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ll.forward(got, level, lmin, w, masks, True, overflow=overflow)
+    assert overflow.item() == 0
+
+    def only(d):
+        return [lmin[i] if i == d else level[i] for i in range(3)]
+
+    want = lr.integer_lifting(q, level, only(0), True).astype(np.int32)
+    t = torch.from_numpy(want).cuda()
+    ll.forward(t, level, lmin, w, masks, True, axis=1)
+    want = lr.integer_lifting(t.cpu().numpy(), level, only(2), True)
+    assert np.array_equal(got.cpu().numpy(), want)
+    ll.inverse(got, level, lmin, w, masks, True, overflow=overflow)
+    assert overflow.item() == 0
+    assert np.array_equal(got.cpu().numpy(), q)
+
+
+@pytest.mark.parametrize("update", [True, False])
+def test_interpolet_overflow_is_flagged(ll, update):
+    """A row alternating between the ends of the int32 range: the finest
+    interpolet detail, INT32_MAX - floor((2 INT32_MIN + 1) / 2), leaves int32
+    and the sweep raises the overflow flag (axis 0 sees constant columns and
+    leaves them unchanged)."""
+    level, lmin = [4, 5], [0, 0]
+    q = np.full([17, 33], np.iinfo(np.int32).max, dtype=np.int32)
+    q[:, 0::2] = np.iinfo(np.int32).min
+    for strided in (False, True):
+        a = q.T.copy() if strided else q
+        lv = level[::-1] if strided else level
+        t = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+        overflow = torch.zeros(1, dtype=torch.int32, device="cuda")
+        ll.forward(t, lv, lmin, None, None, update, overflow=overflow)
+        assert overflow.item() == 1, strided
+    # in range: no flag
+    t = torch.from_numpy(int_grid(np.random.default_rng(37), level, True)).cuda()
+    overflow = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ll.forward(t, level, lmin, None, None, update, overflow=overflow)
+    assert overflow.item() == 0
+
+
 def test_extract_pairs_and_train(ll):
     from paliwa_amd import fields
 
