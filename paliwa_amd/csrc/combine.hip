@@ -11,22 +11,24 @@
 // Work layout.  A prefix p = (k_0..k_{D-2}) with K_p = its largest counted
 // k_{D-1} owns rows_p = prod 2^max(k_d - 1, 0) rows; row r holds the points
 // of the subspaces (p, 0..K_p) at prefix point r, 2^K_p in all, and each
-// component grid containing (p, 0) stores them in one of its rows.  A
-// workgroup covers 1024 positions of one prefix: R = 1024 / 2^K_p whole rows
-// (grouped by subspace, rows outer, m inner) or, for longer rows, 1024
-// positions of one row in heap order (y = 0 -> k = 0; y >= 1 -> k =
-// floor(log2 y) + 1, m = y - 2^(k-1)).  It loops over the grids containing
-// (p, 0), in grid order, with the grid's descriptor in scalar registers; in
-// each iteration every thread whose 4 points lie in the grid (k <= l_{D-1})
-// loads them and accumulates.  So the fold order per point is the grid order,
-// the loads of one iteration cover the block's grid rows together (the
-// strided subsets of the subspaces complete each other's cache lines), four
-// loads per thread are in flight, and the writes are contiguous runs.
+// component grid containing (p, 0) stores them in one of its rows.  Blocks of
+// 1024 positions of one prefix hold R = 1024 / 2^K_p whole rows (grouped by
+// subspace, rows outer, m inner) or, for longer rows, 1024 positions of one
+// row in heap order (y = 0 -> k = 0; y >= 1 -> k = floor(log2 y) + 1,
+// m = y - 2^(k-1)); either way the subspace level never decreases along a
+// block.  A wavefront takes 64 consecutive positions of a block, one per lane,
+// and folds over the grids that hold its smallest level (so the wavefronts of
+// a block's fine levels walk short lists), in grid order: the fold order per
+// point is the grid order.  Wavefronts are dispatched longest list first.
+// The critical path is the longest list (1325 grids for the 4-D 1325-grid
+// set); it is walked kUnroll grids per step with the next step's loads in
+// flight (see combine_gather).
 #include "common.hpp"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cstring>
 #include <map>
 #include <new>
@@ -38,10 +40,21 @@ using namespace paliwa_hip;
 
 constexpr int kCombineDims = PALIWA_COMBINE_MAX_DIMS;
 constexpr int kThreads = 256;
-constexpr int kPerThread = 4;
+constexpr int kWaves = kThreads / 64;
 constexpr int kLogBlock = 10;
 constexpr int64_t kBlockPositions = int64_t(1) << kLogBlock;
-static_assert(kThreads * kPerThread == kBlockPositions, "block size");
+#ifndef PALIWA_COMBINE_UNROLL
+#define PALIWA_COMBINE_UNROLL 8
+#endif
+constexpr int kUnroll = PALIWA_COMBINE_UNROLL; // grids per step
+constexpr int kChunk = 64;                      // term records per chunk
+constexpr int kSteps = kChunk / kUnroll;        // steps per chunk
+static_assert(kChunk % kUnroll == 0, "a step never straddles two chunks");
+// term records read past the end of a list (prefetch of the next chunk, the
+// records of the step after the last): the table ends with this many
+// padding records, so such reads stay inside it
+constexpr int kPadRecords = 2 * kChunk + kUnroll;
+constexpr int kLevelShift = 56; // last-axis level in the pointer word's top byte
 
 struct PrefixDesc {
   int64_t rows;       // points of the prefix subspace
@@ -50,26 +63,32 @@ struct PrefixDesc {
   int32_t k[kCombineDims];
 };
 
-struct BlockJob {
+// One wavefront's work: 64 consecutive positions t0 .. t0+63 of a 1024-point
+// block of a prefix, and the grids (in grid order) holding any of them.
+struct WaveJob {
   int32_t prefix;
   int32_t local;      // block index within the prefix
-  int32_t list_begin; // first term record of the block
+  int32_t t0;         // first position within the block (a multiple of 64)
+  int32_t list_begin; // first term record
   int32_t list_len;
+  int32_t pad;
 };
 
-// A term record: one grid holding a prefix's points, D + 2 words
-// {data pointer, coefficient bits, last-axis level, mult_0..mult_{D-2}}; the
-// grid element of a point is sum_d (2 m_d + 1) mult_d over the prefix axes
+// A term record: one grid holding a prefix's points, D + 1 words
+// {data pointer | last-axis level << 56, coefficient bits, mult_0..mult_{D-2}};
+// the grid element of a point is sum_d (2 m_d + 1) mult_d over the prefix axes
 // (mult_d = stride_d << (l_d - k_d), 0 where k_d = 0) plus, along the last
 // axis, (2 m + 1) << (l - k) for k >= 1 and 0 for k = 0.
 struct CombineArgs {
   PrefixDesc const *prefixes;
   int64_t const *sub_offset; // first sparse element of each subspace
-  int64_t const *records;      // per prefix (and smallest last level), grid order
-  BlockJob const *jobs;        // [nblocks]
-  int32_t const *prefix_of;    // dense prefix-level table -> prefix id or -1
+  int64_t const *records;    // per (prefix, smallest last level), grid order
+  WaveJob const *jobs;       // [njobs], longest grid list first
+  int32_t const *prefix_of;  // dense prefix-level table -> prefix id or -1
   int32_t ndim;
   int32_t nprefix;
+  int32_t njobs;
+  int32_t narrow;                    // records of the narrow layout
   int32_t max_level[kCombineDims];
   int32_t prefix_mult[kCombineDims]; // dense index multipliers of prefix levels
 };
@@ -90,112 +109,190 @@ template <> struct Accumulate<float> {
   }
 };
 
-template <typename T, int D>
+// lane `src`'s 64-bit word, in scalar registers
+__device__ __forceinline__ int64_t lane_word(int64_t v, int src) {
+  uint32_t const lo = static_cast<uint32_t>(
+      __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v)), src));
+  uint32_t const hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(
+      static_cast<int>(static_cast<uint32_t>(static_cast<uint64_t>(v) >> 32)), src));
+  return static_cast<int64_t>((uint64_t(hi) << 32) | lo);
+}
+
+// Global-memory pointer: its loads are global_load (counted by vmcnt only),
+// not flat loads, which also count against lgkmcnt and would make every
+// wait on a record broadcast wait for the grid loads as well.
+template <typename T> using global_ptr = T const __attribute__((address_space(1))) *;
+
+// Words per term record.  Wide records hold each prefix multiplier in a
+// 64-bit word; narrow records (every grid below 2^24 elements, so every
+// element index, odd multiplier and prefix multiplier fits in 24 bits) pack
+// two 32-bit multipliers per word and index with 24-bit multiply-adds.
+template <int D, bool NARROW> constexpr int record_words() {
+  return NARROW ? 2 + D / 2 : D + 1;
+}
+constexpr int64_t kNarrowElements = int64_t(1) << 24;
+
+// The records of one step (kUnroll grids), wave-uniform.
+template <typename T, int D, bool NARROW> struct StepRecords {
+  using Mult = std::conditional_t<NARROW, uint32_t, int64_t>;
+  global_ptr<T> ptr[kUnroll];
+  double coef[kUnroll];
+  int lg[kUnroll];
+  Mult mult[kUnroll][D > 1 ? D - 1 : 1];
+};
+
+// Gather over the grids of each wavefront's list.  Lane = position; the
+// records of a list arrive 64 at a time (lane j holds record j of the chunk)
+// and are broadcast with v_readlane.  Every load is issued unconditionally
+// (positions outside a grid read its element 0 and drop the value), so the
+// loop has no branch inside a chunk and the loads of step s + 1 stay in
+// flight while step s accumulates: the fold over a long list is paced by
+// issue, not by one memory latency per step.
+template <typename T, int D, bool NARROW>
 __global__ void __launch_bounds__(kThreads)
     combine_gather(CombineArgs a, T *__restrict__ sparse) {
-  constexpr int P = D > 1 ? D - 1 : 1; // prefix axes (at least one slot)
-  BlockJob const job = a.jobs[blockIdx.x];
+  using Records = StepRecords<T, D, NARROW>;
+  constexpr int P = D > 1 ? D - 1 : 1;            // prefix axes (at least one slot)
+  constexpr int W = record_words<D, NARROW>();    // words per term record
+  constexpr int U = kUnroll;
+  int const lane = threadIdx.x & 63;
+  int const wj = static_cast<int>(blockIdx.x) * kWaves +
+                 __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  if (wj >= a.njobs)
+    return;
+  WaveJob const job = a.jobs[wj];
   PrefixDesc const &pd = a.prefixes[job.prefix];
   int const K = pd.K;
-  int pk[P];
-#pragma unroll
-  for (int d = 0; d < P; ++d)
-    pk[d] = d < D - 1 ? pd.k[d] : 0;
-  // this thread's points: subspace level k along the last axis, odd
+  // this lane's point: subspace level k along the last axis, odd
   // multipliers 2 m_d + 1 along the prefix axes, output index
-  int k[kPerThread];
-  int64_t col_odd[kPerThread];
-  int64_t o[kPerThread][P];
-  int64_t out[kPerThread];
-#pragma unroll
-  for (int r = 0; r < kPerThread; ++r) {
-    int const t = r * kThreads + threadIdx.x;
-    int64_t row, m;
-    if (K <= kLogBlock) {
-      int const logR = kLogBlock - K;
-      int const u = t >> logR;
-      k[r] = u == 0 ? 0 : 32 - __clz(u);
-      int const logw = k[r] == 0 ? 0 : k[r] - 1;
-      int const within = t - (k[r] == 0 ? 0 : 1 << (logR + k[r] - 1));
-      row = (int64_t(job.local) << logR) + (within >> logw);
-      m = within & ((1 << logw) - 1);
-      if (row >= pd.rows)
-        k[r] = 64; // no point: never inside a grid
-    } else {
-      int const split = K - kLogBlock; // a row spans 2^split workgroups
-      row = int64_t(job.local) >> split;
-      int64_t const y =
-          ((int64_t(job.local) & ((int64_t(1) << split) - 1)) << kLogBlock) + t;
-      k[r] = y == 0 ? 0 : 64 - __clzll(y);
-      m = y == 0 ? 0 : y - (int64_t(1) << (k[r] - 1));
-    }
-    col_odd[r] = 2 * m + 1;
+  int const t = job.t0 + lane;
+  int k;
+  int64_t row, m;
+  if (K <= kLogBlock) {
+    int const logR = kLogBlock - K;
+    int const u = t >> logR;
+    k = u == 0 ? 0 : 32 - __clz(u);
+    int const logw = k == 0 ? 0 : k - 1;
+    int const within = t - (k == 0 ? 0 : 1 << (logR + k - 1));
+    row = (int64_t(job.local) << logR) + (within >> logw);
+    m = within & ((1 << logw) - 1);
+    if (row >= pd.rows)
+      k = 64; // no point: never inside a grid
+  } else {
+    int const split = K - kLogBlock; // a row spans 2^split blocks
+    row = int64_t(job.local) >> split;
+    int64_t const y =
+        ((int64_t(job.local) & ((int64_t(1) << split) - 1)) << kLogBlock) + t;
+    k = y == 0 ? 0 : 64 - __clzll(y);
+    m = y == 0 ? 0 : y - (int64_t(1) << (k - 1));
+  }
+  int64_t const col_odd = 2 * m + 1;
+  int64_t o[P];
+  {
     int64_t rest = row;
 #pragma unroll
-    for (int d = D - 2; d >= 0; --d) {
-      int const sl = shape_log(pk[d]);
-      o[r][d] = 2 * (rest & ((int64_t(1) << sl) - 1)) + 1;
+    for (int d = P - 1; d >= 0; --d) {
+      int const sl = d < D - 1 ? shape_log(pd.k[d]) : 0;
+      o[d] = 2 * (rest & ((int64_t(1) << sl) - 1)) + 1;
       rest >>= sl;
     }
-    out[r] = k[r] <= K ? a.sub_offset[pd.first_sub + k[r]] + (row << shape_log(k[r])) + m : -1;
   }
-  constexpr int W = D + 2;
-  constexpr int U = 4; // grids per step: U * kPerThread loads in flight
-  T s[kPerThread];
-#pragma unroll
-  for (int r = 0; r < kPerThread; ++r)
-    s[r] = T(0);
+  int64_t const out = k <= K ? a.sub_offset[pd.first_sub + k] + (row << shape_log(k)) + m : -1;
+
   int64_t const *rec = a.records + int64_t(job.list_begin) * W;
   int const len = job.list_len;
-  int e = 0;
-  for (; e + U <= len; e += U) {
-    T v[U][kPerThread];
-    int lg[U];
+  int64_t cur[W], nxt[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    cur[w] = rec[int64_t(lane) * W + w];
+
+  auto extract = [&](int64_t const (&ch)[W], int base, Records &r) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      int64_t const *w = rec + int64_t(e + u) * W;
-      T const *data = reinterpret_cast<T const *>(w[0]);
-      lg[u] = static_cast<int>(w[2]);
+      int64_t const pw = lane_word(ch[0], base + u);
+      r.ptr[u] = (global_ptr<T>)(static_cast<uintptr_t>(pw & ((int64_t(1) << kLevelShift) - 1)));
+      r.lg[u] = static_cast<int>(static_cast<uint64_t>(pw) >> kLevelShift);
+      r.coef[u] = __longlong_as_double(lane_word(ch[1], base + u));
 #pragma unroll
-      for (int r = 0; r < kPerThread; ++r) {
-        if (k[r] <= lg[u]) {
-          int64_t idx = k[r] > 0 ? col_odd[r] << (lg[u] - k[r]) : 0;
-#pragma unroll
-          for (int d = 0; d < D - 1; ++d)
-            idx += o[r][d] * w[3 + d];
-          v[u][r] = data[idx];
+      for (int d = 0; d < D - 1; ++d) {
+        if constexpr (NARROW) {
+          uint64_t const word = static_cast<uint64_t>(ch[2 + d / 2]);
+          r.mult[u][d] = static_cast<uint32_t>(__builtin_amdgcn_readlane(
+              static_cast<int>(static_cast<uint32_t>(d % 2 ? word >> 32 : word)), base + u));
+        } else {
+          r.mult[u][d] = lane_word(ch[2 + d], base + u);
         }
       }
     }
+  };
+  // grid e + u holds this lane's point
+  auto inside = [&](Records const &r, int e, int u) {
+    return e + u < len && k <= r.lg[u];
+  };
+  auto issue = [&](Records const &r, int e, T (&v)[U]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      double const c = __longlong_as_double(rec[int64_t(e + u) * W + 1]);
-#pragma unroll
-      for (int r = 0; r < kPerThread; ++r)
-        if (k[r] <= lg[u])
-          s[r] = Accumulate<T>::step(s[r], c, v[u][r]);
-    }
-  }
-  for (; e < len; ++e) {
-    int64_t const *w = rec + int64_t(e) * W;
-    T const *data = reinterpret_cast<T const *>(w[0]);
-    double const c = __longlong_as_double(w[1]);
-    int const lgl = static_cast<int>(w[2]);
-#pragma unroll
-    for (int r = 0; r < kPerThread; ++r) {
-      if (k[r] <= lgl) {
-        int64_t idx = k[r] > 0 ? col_odd[r] << (lgl - k[r]) : 0;
+      bool const in = inside(r, e, u);
+      if constexpr (NARROW) {
+        uint32_t idx = k > 0 ? static_cast<uint32_t>(col_odd) << (in ? r.lg[u] - k : 0) : 0u;
 #pragma unroll
         for (int d = 0; d < D - 1; ++d)
-          idx += o[r][d] * w[3 + d];
-        s[r] = Accumulate<T>::step(s[r], c, data[idx]);
+          idx += __umul24(static_cast<uint32_t>(o[d]), r.mult[u][d]);
+        v[u] = r.ptr[u][in ? idx : 0u];
+      } else {
+        int64_t idx = k > 0 ? col_odd << (in ? r.lg[u] - k : 0) : 0;
+#pragma unroll
+        for (int d = 0; d < D - 1; ++d)
+          idx += o[d] * r.mult[u][d];
+        v[u] = r.ptr[u][in ? idx : 0];
       }
     }
-  }
+  };
+  T acc = T(0);
+  auto consume = [&](Records const &r, int e, T const (&v)[U]) {
 #pragma unroll
-  for (int r = 0; r < kPerThread; ++r)
-    if (out[r] >= 0)
-      sparse[out[r]] = s[r];
+    for (int u = 0; u < U; ++u) {
+      T const next = Accumulate<T>::step(acc, r.coef[u], v[u]);
+      acc = inside(r, e, u) ? next : acc;
+    }
+  };
+
+  Records r0, r1;
+  T v0[U], v1[U];
+  extract(cur, 0, r0);
+  issue(r0, 0, v0);
+  for (int c0 = 0;; c0 += kChunk) {
+    // steps c0, c0 + U, ... of this chunk, unrolled: word w of the next
+    // chunk's records is fetched at step w mod kSteps
+#pragma unroll
+    for (int j = 0; j < kSteps; ++j) {
+      int const e = c0 + j * U;
+#pragma unroll
+      for (int w = j; w < W; w += kSteps)
+        nxt[w] = rec[int64_t(c0 + kChunk + lane) * W + w];
+      if (j + 1 < kSteps)
+        extract(cur, (j + 1) * U, r1);
+      else
+        extract(nxt, 0, r1);
+      issue(r1, e + U, v1);
+      // keep the scheduler from sinking these loads to their uses a step
+      // later, which would put one memory latency back on every step
+      __builtin_amdgcn_sched_barrier(0);
+      consume(r0, e, v0);
+      if (e + U >= len)
+        goto done;
+      r0 = r1;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        v0[u] = v1[u];
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      cur[w] = nxt[w];
+  }
+done:
+  if (out >= 0)
+    sparse[out] = acc;
 }
 
 struct FullExtents {
@@ -250,7 +347,10 @@ __global__ void __launch_bounds__(kThreads)
 template <typename T, int D>
 void launch_gather(CombineArgs const &a, int32_t nblocks, void *sparse,
                    hipStream_t s) {
-  combine_gather<T, D><<<nblocks, kThreads, 0, s>>>(a, static_cast<T *>(sparse));
+  if (a.narrow)
+    combine_gather<T, D, true><<<nblocks, kThreads, 0, s>>>(a, static_cast<T *>(sparse));
+  else
+    combine_gather<T, D, false><<<nblocks, kThreads, 0, s>>>(a, static_cast<T *>(sparse));
 }
 
 template <typename T, int D>
@@ -434,13 +534,20 @@ int paliwa_combine_create(paliwa_combine **plan, int32_t dtype, int32_t ndim,
   c->sub_offsets.push_back(offset);
   c->nsub = static_cast<int32_t>(sub_offset.size());
   c->sparse_points = offset;
-  // workgroups: per prefix, 1024 positions each (whole rows when a row has
-  // at most 1024 points), each with the grids holding its points, in grid
-  // order: the grids containing (p, 0) whose last level reaches the
-  // workgroup's smallest last-axis subspace level
-  std::vector<BlockJob> jobs;
+  // wavefront jobs: per prefix, blocks of 1024 positions (whole rows when a
+  // row has at most 1024 points), 64 positions per wavefront; a wavefront's
+  // list holds the grids containing (p, 0) whose last level reaches its
+  // smallest last-axis subspace level, in grid order
+  std::vector<WaveJob> jobs;
   std::vector<int64_t> records;
-  int64_t const W = ndim + 2;
+  bool narrow = true;
+  for (int32_t g = 0; g < ngrids; ++g) {
+    int64_t n = 1;
+    for (int d = 0; d < ndim; ++d)
+      n *= extents[int64_t(g) * ndim + d];
+    narrow = narrow && n < kNarrowElements;
+  }
+  int64_t const W = narrow ? 2 + ndim / 2 : ndim + 1;
   std::vector<int32_t> holders;
   std::vector<int64_t> grid_stride(int64_t(ngrids) * ndim);
   for (int32_t g = 0; g < ngrids; ++g) {
@@ -448,6 +555,11 @@ int paliwa_combine_create(paliwa_combine **plan, int32_t dtype, int32_t ndim,
     for (int d = ndim - 1; d >= 0; --d) {
       grid_stride[int64_t(g) * ndim + d] = stride;
       stride *= extents[int64_t(g) * ndim + d];
+    }
+    uint64_t const ptr = reinterpret_cast<uintptr_t>(data[g]);
+    if ((ptr >> kLevelShift) != 0) {
+      destroy(c);
+      return fail(PALIWA_ERR_UNSUPPORTED, "combine: grid address above 2^56");
     }
   }
   for (size_t p = 0; p < prefixes.size(); ++p) {
@@ -465,52 +577,86 @@ int paliwa_combine_create(paliwa_combine **plan, int32_t dtype, int32_t ndim,
       if (inside)
         holders.push_back(g);
     }
+    // list of the holders whose last level reaches kmin, built once per kmin
+    std::vector<std::pair<int32_t, int32_t>> list_of(pd.K + 1, {-1, 0});
+    auto list_for = [&](int kmin) -> std::pair<int32_t, int32_t> {
+      if (list_of[kmin].first >= 0)
+        return list_of[kmin];
+      int64_t const begin = int64_t(records.size()) / W;
+      for (int32_t g : holders) {
+        int32_t const *l = levels + int64_t(g) * ndim;
+        if (l[ndim - 1] < kmin)
+          continue;
+        uint64_t const ptr = reinterpret_cast<uintptr_t>(data[g]);
+        records.push_back(static_cast<int64_t>(ptr | (uint64_t(l[ndim - 1]) << kLevelShift)));
+        int64_t word = 0;
+        std::memcpy(&word, coefficients + g, sizeof(double));
+        records.push_back(word);
+        int64_t pair = 0;
+        for (int d = 0; d + 1 < ndim; ++d) {
+          int64_t const mult =
+              pd.k[d] > 0 ? grid_stride[int64_t(g) * ndim + d] << (l[d] - pd.k[d]) : 0;
+          if (!narrow) {
+            records.push_back(mult);
+          } else if (d % 2 == 0) {
+            pair = mult;
+            if (d + 2 >= ndim) // last multiplier: the word's high half stays 0
+              records.push_back(pair);
+          } else {
+            records.push_back(pair | (mult << 32));
+          }
+        }
+      }
+      int64_t const len = int64_t(records.size()) / W - begin;
+      if (begin > INT32_MAX - kPadRecords)
+        return {-1, -1};
+      list_of[kmin] = {static_cast<int32_t>(begin), static_cast<int32_t>(len)};
+      return list_of[kmin];
+    };
     int64_t const nb = pd.K <= kLogBlock
                            ? (rows + (int64_t(1) << (kLogBlock - pd.K)) - 1) >> (kLogBlock - pd.K)
                            : rows << (pd.K - kLogBlock);
-    if (int64_t(jobs.size()) + nb > INT32_MAX) {
-      destroy(c);
-      return fail(PALIWA_ERR_UNSUPPORTED, "combine: sparse grid too large");
-    }
-    int32_t whole_begin = -1; // list shared by the workgroups starting a row
     for (int64_t b = 0; b < nb; ++b) {
-      int kmin = 0;
-      if (pd.K > kLogBlock) {
-        int64_t const y0 = (b & ((int64_t(1) << (pd.K - kLogBlock)) - 1)) << kLogBlock;
-        kmin = y0 == 0 ? 0 : 64 - __builtin_clzll(static_cast<unsigned long long>(y0));
-      }
-      BlockJob job{static_cast<int32_t>(p), static_cast<int32_t>(b), 0, 0};
-      if (kmin == 0 && whole_begin >= 0) {
-        job.list_begin = whole_begin;
-        job.list_len = static_cast<int32_t>(holders.size());
-      } else {
-        if (int64_t(records.size()) / W + int64_t(holders.size()) > INT32_MAX) {
+      for (int32_t t0 = 0; t0 < kBlockPositions; t0 += 64) {
+        // smallest subspace level among the wavefront's points
+        int kmin = -1;
+        for (int32_t t = t0; t < t0 + 64 && kmin < 0; ++t) {
+          if (pd.K <= kLogBlock) {
+            int const logR = kLogBlock - pd.K;
+            int const u = t >> logR;
+            int const k = u == 0 ? 0 : 32 - __builtin_clz(static_cast<unsigned>(u));
+            int const logw = k == 0 ? 0 : k - 1;
+            int const within = t - (k == 0 ? 0 : 1 << (logR + k - 1));
+            if ((b << logR) + (within >> logw) < rows)
+              kmin = k;
+          } else {
+            int const split = pd.K - kLogBlock;
+            int64_t const y = ((b & ((int64_t(1) << split) - 1)) << kLogBlock) + t;
+            kmin = y == 0 ? 0 : 64 - __builtin_clzll(static_cast<unsigned long long>(y));
+          }
+        }
+        if (kmin < 0)
+          continue; // no point in this wavefront
+        auto const [begin, len] = list_for(kmin);
+        if (begin < 0 || b > INT32_MAX || jobs.size() >= size_t(INT32_MAX) / kWaves) {
           destroy(c);
-          return fail(PALIWA_ERR_UNSUPPORTED, "combine: too many grid lists");
+          return fail(PALIWA_ERR_UNSUPPORTED, "combine: sparse grid too large");
         }
-        job.list_begin = static_cast<int32_t>(records.size() / W);
-        for (int32_t g : holders) {
-          int32_t const *l = levels + int64_t(g) * ndim;
-          if (l[ndim - 1] < kmin)
-            continue;
-          int64_t word = 0;
-          void const *ptr = data[g];
-          std::memcpy(&word, &ptr, sizeof(ptr));
-          records.push_back(word);
-          std::memcpy(&word, coefficients + g, sizeof(double));
-          records.push_back(word);
-          records.push_back(l[ndim - 1]);
-          for (int d = 0; d + 1 < ndim; ++d)
-            records.push_back(pd.k[d] > 0 ? grid_stride[int64_t(g) * ndim + d] << (l[d] - pd.k[d]) : 0);
-        }
-        job.list_len = static_cast<int32_t>(records.size() / W) - job.list_begin;
-        if (kmin == 0)
-          whole_begin = job.list_begin;
+        jobs.push_back({static_cast<int32_t>(p), static_cast<int32_t>(b), t0, begin, len, 0});
       }
-      jobs.push_back(job);
     }
   }
-  int64_t const nblocks = static_cast<int64_t>(jobs.size());
+  // padding records: a valid pointer, never inside (they lie past every list)
+  for (int i = 0; i < kPadRecords; ++i) {
+    records.push_back(static_cast<int64_t>(reinterpret_cast<uintptr_t>(data[0])));
+    for (int64_t w = 1; w < W; ++w)
+      records.push_back(0);
+  }
+  // longest lists first: they set the gather's critical path
+  std::stable_sort(jobs.begin(), jobs.end(), [](WaveJob const &x, WaveJob const &y) {
+    return x.list_len > y.list_len;
+  });
+  int64_t const njobs = static_cast<int64_t>(jobs.size());
   // dense prefix table
   std::vector<int32_t> prefix_of(dense_prefixes, -1);
   int32_t mult[kCombineDims] = {};
@@ -548,7 +694,7 @@ int paliwa_combine_create(paliwa_combine **plan, int32_t dtype, int32_t ndim,
   a.prefixes = reinterpret_cast<PrefixDesc const *>(base + at_prefixes);
   a.sub_offset = reinterpret_cast<int64_t const *>(base + at_subs);
   a.records = reinterpret_cast<int64_t const *>(base + at_records);
-  a.jobs = reinterpret_cast<BlockJob const *>(base + at_jobs);
+  a.jobs = reinterpret_cast<WaveJob const *>(base + at_jobs);
   a.prefix_of = reinterpret_cast<int32_t const *>(base + at_dense);
   a.ndim = ndim;
   a.nprefix = static_cast<int32_t>(prefixes.size());
@@ -556,7 +702,9 @@ int paliwa_combine_create(paliwa_combine **plan, int32_t dtype, int32_t ndim,
     a.max_level[d] = maximum_level[d];
     a.prefix_mult[d] = mult[d];
   }
-  c->nblocks = static_cast<int32_t>(nblocks);
+  a.njobs = static_cast<int32_t>(njobs);
+  a.narrow = narrow ? 1 : 0;
+  c->nblocks = static_cast<int32_t>((njobs + kWaves - 1) / kWaves);
   *plan = c;
   return PALIWA_OK;
 }
