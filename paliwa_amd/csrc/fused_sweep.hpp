@@ -516,6 +516,18 @@ __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
 #pragma unroll
       for (int j = 0; j < K; ++j)
         v[j] = lds[lds_index<K, G, STRIDED>(li, gi * K + j, LP)];
+      if constexpr (kInt) {
+        // lines past the grid's end (last tile) hold stale LDS words; they
+        // are never stored, but their arithmetic must not raise the overflow
+        // flag: run them on zeros
+        TileRef const ref = src(t);
+        int64_t const lines = STRIDED ? ref.A * ref.B : ref.A;
+        if (ref.local * TB + li >= lines) {
+#pragma unroll
+          for (int j = 0; j < K; ++j)
+            v[j] = T(0);
+        }
+      }
       compute(v);
 #pragma unroll
       for (int j = 0; j < K; ++j)
