@@ -141,3 +141,26 @@ def test_combination_technique_set_1325_grids(pw):
     assert np.array_equal(sparse.cpu().numpy(), want)
     comb.close()
     batch.close()
+
+
+def test_wide_records_for_large_grids(pw):
+    """A grid of 2^24 elements or more makes the plan keep 64-bit prefix
+    multipliers and 64-bit index arithmetic (the narrow 24-bit form holds
+    only smaller grids): a 257^3 grid with three of its coarser neighbours,
+    against the oracle, both subspace rules."""
+    from paliwa_amd.combine import SparseGridCombination
+
+    levels = [(8, 8, 8), (8, 8, 7), (8, 7, 8), (7, 8, 8)]
+    coefs = [1.0, -0.375, -0.375, -0.25]
+    M = [8, 8, 8]
+    rng = np.random.default_rng(7)
+    host = [rng.standard_normal([(1 << l) + (1 if i == 0 else 0) for l in lv])
+            for i, lv in enumerate(levels)]
+    assert host[0].size >= 1 << 24
+    grids = [torch.from_numpy(h).cuda() for h in host]
+    for min_count in (1, 2):
+        comb = SparseGridCombination(grids, levels, coefs, M, min_count=min_count)
+        want, subs, offs = oc.gather(host, levels, coefs, min_count)
+        assert comb.subspaces == subs
+        assert np.array_equal(comb.gather().cpu().numpy(), want), min_count
+        comb.close()
