@@ -34,7 +34,8 @@ struct paliwa_batch {
     int32_t dim = 0, L = 0, m = 0;
     bool strided = false;
     bool fused = false;
-    int64_t tiles = 0;               // fused groups
+    bool lds_lines = false;          // longer lines: one batched LDS-line launch
+    int64_t tiles = 0;               // fused groups: tiles; LDS-line groups: lines
     BatchGrid *grids_dev = nullptr;  // fused groups: descriptors on the device
     int32_t *tile_grid_dev = nullptr;
     std::vector<int32_t> members;    // grid indices
@@ -72,6 +73,19 @@ int launch_group(paliwa_batch &b, paliwa_batch::Group const &grp,
     hipError_t const err = launch_fused_batched(
         b.dtype, p, grp.strided, ManyGrids{grp.grids_dev, grp.tile_grid_dev},
         grp.tiles, stream);
+    if (err != hipSuccess)
+      return fail(PALIWA_ERR_HIP, std::string("HIP: ") + hipGetErrorString(err));
+    return PALIWA_OK;
+  }
+  if (grp.lds_lines) {
+    SweepParams p{};
+    p.m = grp.m;
+    p.L = grp.L;
+    set_filters(p, filters, nfilters);
+    set_named_levels(p, grp.L, b.lmin[grp.dim], forward);
+    classify(p);
+    hipError_t const err = launch_lds_line_batched(
+        b.dtype, p, ManyGrids{grp.grids_dev, grp.tile_grid_dev}, grp.tiles, stream);
     if (err != hipSuccess)
       return fail(PALIWA_ERR_HIP, std::string("HIP: ") + hipGetErrorString(err));
     return PALIWA_OK;
@@ -116,9 +130,13 @@ int launch_sequence(paliwa_batch &b, WaveletEntry const &w, bool forward,
       for (size_t t = 0; t < b.lanes.size() && rc == PALIWA_OK; ++t)
         if (used[t])
           rc = hip(hipStreamWaitEvent(b.lanes[t], b.fork, 0));
-      for (size_t k = i; k < j && rc == PALIWA_OK; ++k)
-        rc = launch_group(b, b.groups[k], filters, nfilters, forward,
-                          b.lanes[b.lane_of[k]]);
+      // long-line groups first: few workgroups each, latency-bound, so they
+      // run beside the large groups instead of after them
+      for (int pass = 0; pass < 2 && rc == PALIWA_OK; ++pass)
+        for (size_t k = i; k < j && rc == PALIWA_OK; ++k)
+          if (b.groups[k].lds_lines == (pass == 0))
+            rc = launch_group(b, b.groups[k], filters, nfilters, forward,
+                              b.lanes[b.lane_of[k]]);
       for (size_t t = 0; t < b.lanes.size() && rc == PALIWA_OK; ++t)
         if (used[t]) {
           rc = hip(hipEventRecord(b.joins[t], b.lanes[t]));
@@ -269,10 +287,12 @@ int paliwa_batch_create(paliwa_batch **plan, int32_t dtype, int32_t ndim,
       grp.strided = d + 1 < ndim;
       grp.members = members;
       grp.fused = (int64_t(1) << grp.L) <= fused_max_period(dtype);
+      grp.lds_lines = !grp.fused && (int64_t(1) << grp.L) <= lds_line_max_period(dtype);
       std::vector<BatchGrid> gd;
       std::vector<int32_t> tg;
-      if (grp.fused) {
-        int64_t const tb = fused_tile_lines(dtype, grp.L, grp.strided);
+      if (grp.fused || grp.lds_lines) {
+        // fused: tiles of fused_tile_lines lines; LDS-line: one line per tile
+        int64_t const tb = grp.fused ? fused_tile_lines(dtype, grp.L, grp.strided) : 1;
         for (size_t k = 0; k < members.size(); ++k) {
           int32_t const g = members[k];
           SweepParams const p =

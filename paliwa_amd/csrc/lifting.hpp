@@ -154,6 +154,11 @@ hipError_t launch_fused_batched_f32(SweepParams const &p, bool strided,
                                     hipStream_t stream);
 hipError_t launch_lds_line(int dtype, SweepParams const &p,
                            hipStream_t stream);
+// Batched LDS-line sweep: `lines` lines of the grids in `grids` (one "tile"
+// per line), all with line period 2^p.L and stored extent p.m along the axis.
+hipError_t launch_lds_line_batched(int dtype, SweepParams const &p,
+                                   ManyGrids grids, int64_t lines,
+                                   hipStream_t stream);
 hipError_t launch_per_pass(int dtype, SweepParams const &p,
                            hipStream_t stream);
 

@@ -47,44 +47,68 @@ hipError_t allow_lds(Kernel kernel, size_t bytes) {
 // ---------------------------------------------------------------------------
 // lds_line_sweep: one line per workgroup (grid-stride over lines).
 
+// Every level and pass of the sweep on one line held in LDS; `base` points at
+// the line's first point, consecutive points are `B` elements apart.
+template <typename T>
+__device__ __forceinline__ void sweep_line(SweepParams const &p, T *base,
+                                           int64_t B, T *line) {
+  int const tid = threadIdx.x;
+  int const m = p.m;
+  int const n = 1 << p.L;
+  for (int r = tid; r < n; r += kThreads)
+    line[r] = base[r * B];
+  __syncthreads();
+  for (int l = 0; l < p.nlevels; ++l) {
+    int const s = 1 << p.shift[l];
+    for (int f = 0; f < p.npass; ++f) {
+      int const odd = p.offset[f];
+      T const c0 = static_cast<T>(p.coef[f][0]);
+      T const c1 = static_cast<T>(p.coef[f][1]);
+      T const c2 = static_cast<T>(p.coef[f][2]);
+      int const count = (n - odd * s + 2 * s - 1) / (2 * s);
+      for (int k = tid; k < count; k += kThreads) {
+        int const x = odd * s + 2 * s * k;
+        int const lo = x == 0 ? n - s : x - s;
+        int const hi = x + s == n ? 0 : x + s;
+        line[x] = lift(c0, line[lo], c1, line[x], c2, line[hi]);
+      }
+      __syncthreads();
+    }
+  }
+  for (int r = tid; r < n; r += kThreads)
+    base[r * B] = line[r];
+  if (m != n && tid == 0)
+    base[static_cast<int64_t>(n) * B] = line[0];
+  __syncthreads();
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void lds_line_sweep(SweepParams p) {
   extern __shared__ unsigned char smem_raw[];
   T *const line = reinterpret_cast<T *>(smem_raw);
-  int const tid = threadIdx.x;
-  int const m = p.m;
-  int const n = 1 << p.L;
   T *const g = static_cast<T *>(p.data);
   int64_t const lines = p.A * p.B;
   for (int64_t ln = blockIdx.x; ln < lines; ln += gridDim.x) {
     int64_t const a = ln / p.B;
     int64_t const b = ln - a * p.B;
-    T *const base = g + a * m * p.B + b;
-    for (int r = tid; r < n; r += kThreads)
-      line[r] = base[r * p.B];
-    __syncthreads();
-    for (int l = 0; l < p.nlevels; ++l) {
-      int const s = 1 << p.shift[l];
-      for (int f = 0; f < p.npass; ++f) {
-        int const odd = p.offset[f];
-        T const c0 = static_cast<T>(p.coef[f][0]);
-        T const c1 = static_cast<T>(p.coef[f][1]);
-        T const c2 = static_cast<T>(p.coef[f][2]);
-        int const count = (n - odd * s + 2 * s - 1) / (2 * s);
-        for (int k = tid; k < count; k += kThreads) {
-          int const x = odd * s + 2 * s * k;
-          int const lo = x == 0 ? n - s : x - s;
-          int const hi = x + s == n ? 0 : x + s;
-          line[x] = lift(c0, line[lo], c1, line[x], c2, line[hi]);
-        }
-        __syncthreads();
-      }
-    }
-    for (int r = tid; r < n; r += kThreads)
-      base[r * p.B] = line[r];
-    if (m != n && tid == 0)
-      base[static_cast<int64_t>(n) * p.B] = line[0];
-    __syncthreads();
+    sweep_line(p, g + a * p.m * p.B + b, p.B, line);
+  }
+}
+
+// The lines of many grids with equal period and stored extent along the axis
+// in one launch (a batched plan's long-line groups): line t of the launch is
+// line t - tile_base of grid tile_grid[t].
+template <typename T>
+__global__ __launch_bounds__(kThreads) void lds_line_batched(SweepParams p,
+                                                             ManyGrids grids,
+                                                             int64_t lines) {
+  extern __shared__ unsigned char smem_raw[];
+  T *const line = reinterpret_cast<T *>(smem_raw);
+  for (int64_t t = blockIdx.x; t < lines; t += gridDim.x) {
+    TileRef const ref = grids(t);
+    int64_t const a = ref.local / ref.B;
+    int64_t const b = ref.local - a * ref.B;
+    sweep_line(p, static_cast<T *>(ref.data) + a * p.m * ref.B + b, ref.B, line);
   }
 }
 
@@ -170,6 +194,21 @@ hipError_t launch_lds_line_typed(SweepParams const &p, hipStream_t stream) {
   uint32_t const grid =
       static_cast<uint32_t>(std::min<int64_t>(lines, 1 << 20));
   hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), bytes, stream, p);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_lds_line_batched_typed(SweepParams const &p, ManyGrids grids,
+                                         int64_t lines, hipStream_t stream) {
+  size_t const bytes = sizeof(T) * (size_t(1) << p.L);
+  auto kernel = lds_line_batched<T>;
+  hipError_t err = allow_lds(kernel, bytes);
+  if (err != hipSuccess)
+    return err;
+  uint32_t const grid =
+      static_cast<uint32_t>(std::min<int64_t>(lines, 1 << 20));
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), bytes, stream, p,
+                     grids, lines);
   return hipGetLastError();
 }
 
@@ -267,6 +306,14 @@ hipError_t launch_per_pass(int dtype, SweepParams const &p,
                            hipStream_t stream) {
   return dtype == PALIWA_FLOAT64 ? launch_per_pass_typed<double>(p, stream)
                                  : launch_per_pass_typed<float>(p, stream);
+}
+
+hipError_t launch_lds_line_batched(int dtype, SweepParams const &p,
+                                   ManyGrids grids, int64_t lines,
+                                   hipStream_t stream) {
+  return dtype == PALIWA_FLOAT64
+             ? launch_lds_line_batched_typed<double>(p, grids, lines, stream)
+             : launch_lds_line_batched_typed<float>(p, grids, lines, stream);
 }
 
 } // namespace paliwa_hip

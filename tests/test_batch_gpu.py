@@ -63,6 +63,36 @@ def test_batch_matches_oracle(pw, dtype, image):
             batch.close()
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batch_long_lines(pw, dtype):
+    """Lines longer than the fused kernel holds (2^12 and 2^13 float64, 2^13
+    float32) go through one batched LDS-line launch per group: every grid
+    against the oracle, all wavelets, with and without graphs, both
+    directions."""
+    levels = [(12, 1), (12, 2), (13, 1), (1, 13), (2, 12), (3, 12), (11, 2)]
+    lmin, lmax = [0, 1], [13, 13]
+    arrays = make_grids(levels, True, dtype, seed=23)
+    for name in ("hat", "biorthogonal", "fullweighting"):
+        for graphs in (True, False):
+            grids = [torch.from_numpy(a).cuda() for a in arrays]
+            batch = pw.CombinationBatch(grids, levels, lmin, lmax)
+            batch.use_graphs(graphs)
+            batch.hierarchize(name)
+            torch.cuda.synchronize()
+            wants = []
+            for a, g, l in zip(arrays, grids, levels):
+                want = a.copy()
+                ob.hierarchize(want, list(l), lmin, lmax, name)
+                wants.append(want)
+                assert np.array_equal(g.cpu().numpy(), want), (name, l, graphs)
+            batch.dehierarchize(name)
+            torch.cuda.synchronize()
+            for want, g, l in zip(wants, grids, levels):
+                ob.dehierarchize(want, list(l), lmin, lmax, name)
+                assert np.array_equal(g.cpu().numpy(), want), (name, l, graphs)
+            batch.close()
+
+
 def test_batch_replays_graph_repeatedly(pw):
     """Graph replays interleaved with eager work: every replay must equal the
     eager result."""
