@@ -8,7 +8,7 @@ Pipeline:
     q, scale = quantize(field)                        # int32 fixed point
     pairs = extract_pairs(training_grids, level)      # from the interpolet transform
     weights, losses = train(pairs)                    # hand-written fwd/bwd/Adam kernels
-    masks = choose_masks(q, level, lmin, weights)     # MLP only where it lowers L1
+    masks, _ = choose_masks(q, level, lmin, weights)  # MLP only where it lowers L1
     forward(q, ...); inverse(q, ...)                  # exact integer round trip
 """
 from __future__ import annotations
