@@ -15,7 +15,11 @@ Also reported: the dominant kernel's achieved bandwidth against the 8 TB/s
 HBM3E peak (HIP events around every launch in the timed region), a cold
 variant with the 256 MB Infinity Cache flushed before every sweep, and the
 CPU baseline: the same transform by the C++/OpenMP restatement of the
-reference's schedules on the host cores.
+reference's schedules on the host cores.  Further sections: float32 storage,
+2-D and 6-D workloads per axis, the 1325-grid combination-technique set
+(batched vs per-grid, then the combine gather), the learned lifting workload,
+a slab's partial-output transform (one shard of a slab decomposition, with its
+ghost planes) and raw dump / load throughput.
 """
 from __future__ import annotations
 
@@ -64,6 +68,88 @@ def copy_probe(torch):
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
                       "launches": 5}))
+
+
+def partial_benchmark(torch, ranks=8, reps=20):
+    """One rank's share of a 256^3 float64 grid cut into `ranks` slabs along
+    axis 0 (the shard of a slab decomposition): the partial-output forward
+    transform of axis 0 on the slab and its ghost planes (the step after a
+    ghost exchange), for three dependency depths -- hat (one ghost plane),
+    biorthogonal down to level 4, biorthogonal down to level 0 (whose
+    dependency cone is the whole axis) -- beside the whole-grid axis-0 sweep."""
+    import paliwa_amd as pw
+    from paliwa_amd.partial import PartialTransform
+
+    L = 8
+    P = 1 << L
+    n = P // ranks
+    plane = P * P * 8
+    block = torch.randn(n, P, P, dtype=torch.float64, device="cuda")
+    full = torch.randn(P, P, P, dtype=torch.float64, device="cuda")
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    cases = {}
+    for name, lmin in (("hat", 0), ("biorthogonal", 4), ("biorthogonal", 0)):
+        plan = PartialTransform([n, P, P], 0, L, list(range(L, lmin, -1)),
+                                pw.wavelet_filters(name), local_start=0,
+                                output_start=0, output_count=n)
+        g = len(plan.ghost_indices)
+        ghosts = torch.randn(max(g, 1), P, P, dtype=torch.float64, device="cuda")[:g]
+        ms = timed(lambda: plan(block, ghosts if g else None))
+        full_ms = timed(lambda: pw.hierarchize_in(0, full, [L] * 3, [lmin] * 3, [L] * 3, name))
+        moved = (n + g) * plane + n * plane  # inputs read, outputs written
+        cases[f"{name}_lmin{lmin}"] = {
+            "ghost_planes": g,
+            "ms": round(ms, 4),
+            "GBps": round(moved / (ms * 1e-3) / 1e9, 1),
+            "full_axis_sweep_ms": round(full_ms, 4),
+            "full_axis_sweep_GBps": round(2 * P ** 3 * 8 / (full_ms * 1e-3) / 1e9, 1),
+        }
+        plan.close()
+    return {"workload": f"axis 0 of a 256^3 float64 grid in {ranks} slabs: slab 0 "
+                        f"({n} planes) and its ghost planes, forward transform",
+            "cases": cases}
+
+
+def rawio_benchmark(torch):
+    """Raw dump and load (the reference's dump_chunk_span_to_binary_file
+    format) of a 257^3 float64 grid through a temporary file: device-host copy
+    plus file I/O, bounded by PCIe and the host's file system, not by HBM."""
+    import tempfile
+    import time
+
+    from paliwa_amd import rawio
+
+    grid = torch.randn(257, 257, 257, dtype=torch.float64, device="cuda")
+    nbytes = grid.numel() * 8
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "full_grid_8_8_8.raw")
+        rawio.dump(grid, path)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rawio.dump(grid, path)
+        torch.cuda.synchronize()
+        dump_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        back = rawio.load(path, grid.shape, torch.float64)
+        torch.cuda.synchronize()
+        load_s = time.perf_counter() - t0
+        same = bool(torch.equal(back, grid))
+    return {"workload": "257^3 float64 raw dump and load (temporary file)",
+            "bytes": nbytes, "dump_GBps": round(nbytes / dump_s / 1e9, 2),
+            "load_GBps": round(nbytes / load_s / 1e9, 2), "round_trip_equal": same}
 
 
 def cpu_baseline(grid_cpu, budget_s):
@@ -531,6 +617,8 @@ def main():
         record["ct"] = ct_benchmark(torch, pw, fields)
     if not args.no_learned:
         record["learned"] = learned_benchmark()
+    record["partial"] = partial_benchmark(torch)
+    record["rawio"] = rawio_benchmark(torch)
     if not args.no_cpu_baseline:
         record["cpu_baseline"] = cpu_baseline(original.cpu().numpy(), args.cpu_seconds)
     print(json.dumps(record), flush=True)
