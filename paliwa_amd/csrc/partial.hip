@@ -49,10 +49,18 @@ struct PartialArgs {
   int32_t npasses;
   int32_t W;      // window slots
   int32_t period; // P: slot arithmetic is mod P
-  int32_t TB;     // lines per workgroup
+  int32_t TB;     // lines per workgroup (a power of two dividing kThreads)
+  int32_t logTB;
+  uint32_t magic_W, magic_out; // e / W, e / out_count by multiply-high (B == 1)
   int64_t lines, B, m_local, m_ghost;
   int32_t out_slot, out_local, out_count;
 };
+
+// e / d from the host's magic = 2^32 / d rounded up (exact for the e and d
+// here); magic 0 stands for d = 1, whose 2^32 does not fit the word.
+__device__ __forceinline__ int divide(int e, uint32_t magic) {
+  return magic != 0 ? static_cast<int>(__umulhi(static_cast<uint32_t>(e), magic)) : e;
+}
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
@@ -60,23 +68,48 @@ __global__ void __launch_bounds__(kThreads)
   extern __shared__ unsigned char smem[];
   T *t = reinterpret_cast<T *>(smem); // [W][TB]
   int const TB = a.TB;
+  int const logTB = a.logTB;
   int64_t const line0 = int64_t(blockIdx.x) * TB;
-  int const n = a.W * TB;
-  for (int idx = threadIdx.x; idx < n; idx += kThreads) {
-    // along the line when lines are contiguous rows, across lines otherwise
-    int const s = a.B == 1 ? idx % a.W : idx / TB;
-    int const tb = a.B == 1 ? idx / a.W : idx % TB;
+  if (a.B > 1) {
+    // across lines: thread -> (line tb, slots s0, s0 + rows, ...), so each
+    // thread's line and its base pointers are computed once
+    int const tb = threadIdx.x & (TB - 1);
+    int const rows = kThreads >> logTB;
+    int const s0 = threadIdx.x >> logTB;
     int64_t const line = line0 + tb;
-    T v = T(0);
-    if (line < a.lines) {
-      int64_t const ai = line / a.B, bi = line - (line / a.B) * a.B;
+    bool const valid = line < a.lines;
+    int64_t const ai = valid ? line / a.B : 0;
+    int64_t const bi = valid ? line - ai * a.B : 0;
+    T *const lbase = local + ai * a.m_local * a.B + bi;
+    T const *const gbase = ghosts + ai * a.m_ghost * a.B + bi;
+    for (int s = s0; s < a.W; s += rows) {
+      T v = T(0);
       int32_t const src = a.slot_source[s];
-      if (src >= 0)
-        v = local[(ai * a.m_local + src) * a.B + bi];
-      else if (src != kNoSource)
-        v = ghosts[(ai * a.m_ghost + (-src - 1)) * a.B + bi];
+      if (valid) {
+        if (src >= 0)
+          v = lbase[src * a.B];
+        else if (src != kNoSource)
+          v = gbase[int64_t(-src - 1) * a.B];
+      }
+      t[s * TB + tb] = v;
     }
-    t[s * TB + tb] = v;
+  } else {
+    // along lines: consecutive threads read consecutive slots of a line
+    int const n = a.W * TB;
+    for (int idx = threadIdx.x; idx < n; idx += kThreads) {
+      int const tb = divide(idx, a.magic_W);
+      int const s = idx - tb * a.W;
+      int64_t const line = line0 + tb;
+      T v = T(0);
+      if (line < a.lines) {
+        int32_t const src = a.slot_source[s];
+        if (src >= 0)
+          v = local[line * a.m_local + src];
+        else if (src != kNoSource)
+          v = ghosts[line * a.m_ghost + (-src - 1)];
+      }
+      t[s * TB + tb] = v;
+    }
   }
   __syncthreads();
   for (int p = 0; p < a.npasses; ++p) {
@@ -84,9 +117,9 @@ __global__ void __launch_bounds__(kThreads)
     T const c0 = static_cast<T>(ps.c[0]);
     T const c1 = static_cast<T>(ps.c[1]);
     T const c2 = static_cast<T>(ps.c[2]);
-    int const work = ps.count * TB;
+    int const work = ps.count << logTB;
     for (int idx = threadIdx.x; idx < work; idx += kThreads) {
-      int const i = idx / TB, tb = idx - (idx / TB) * TB;
+      int const i = idx >> logTB, tb = idx & (TB - 1);
       // slots mod P (inside the window by the plan)
       int x = ps.first + i * ps.step;
       if (x >= a.period)
@@ -105,21 +138,40 @@ __global__ void __launch_bounds__(kThreads)
     }
     __syncthreads();
   }
-  int const outs = a.out_count * TB;
-  for (int idx = threadIdx.x; idx < outs; idx += kThreads) {
-    int const i = a.B == 1 ? idx % a.out_count : idx / TB;
-    int const tb = a.B == 1 ? idx / a.out_count : idx % TB;
+  if (a.B > 1) {
+    int const tb = threadIdx.x & (TB - 1);
+    int const rows = kThreads >> logTB;
     int64_t const line = line0 + tb;
     if (line >= a.lines)
-      continue;
-    int64_t const ai = line / a.B, bi = line - (line / a.B) * a.B;
-    int s = a.out_slot + i; // mod P, inside the window
-    if (s >= a.period)
-      s -= a.period;
-    int64_t j = a.out_local + i; // wraps only when the local run is the whole period
-    if (j >= a.m_local)
-      j -= a.m_local;
-    local[(ai * a.m_local + j) * a.B + bi] = t[s * TB + tb];
+      return;
+    int64_t const ai = line / a.B;
+    int64_t const bi = line - ai * a.B;
+    T *const lbase = local + ai * a.m_local * a.B + bi;
+    for (int i = threadIdx.x >> logTB; i < a.out_count; i += rows) {
+      int s = a.out_slot + i; // mod P, inside the window
+      if (s >= a.period)
+        s -= a.period;
+      int64_t j = a.out_local + i; // wraps only when the local run is the whole period
+      if (j >= a.m_local)
+        j -= a.m_local;
+      lbase[j * a.B] = t[s * TB + tb];
+    }
+  } else {
+    int const outs = a.out_count * TB;
+    for (int idx = threadIdx.x; idx < outs; idx += kThreads) {
+      int const tb = divide(idx, a.magic_out);
+      int const i = idx - tb * a.out_count;
+      int64_t const line = line0 + tb;
+      if (line >= a.lines)
+        continue;
+      int s = a.out_slot + i;
+      if (s >= a.period)
+        s -= a.period;
+      int64_t j = a.out_local + i;
+      if (j >= a.m_local)
+        j -= a.m_local;
+      local[line * a.m_local + j] = t[s * TB + tb];
+    }
   }
 }
 
@@ -305,10 +357,22 @@ int paliwa_partial_create(paliwa_partial **plan, int32_t dtype, int32_t ndim,
   a.out_slot = static_cast<int32_t>(slot_of(output_start));
   a.out_local = static_cast<int32_t>(out_local);
   a.out_count = static_cast<int32_t>(output_count);
-  // lines per workgroup: as many windows as fit in 32 KiB, at most 64
+  // lines per workgroup: as many windows as fit in 32 KiB, at most 64, a
+  // power of two (so that it divides the workgroup size)
   int64_t tb = std::max<int64_t>(1, (32 * 1024) / (W * elem));
   tb = std::min<int64_t>({tb, 64, a.lines});
+  int log_tb = 0;
+  while ((int64_t(2) << log_tb) <= tb)
+    ++log_tb;
+  tb = int64_t(1) << log_tb;
   a.TB = static_cast<int32_t>(tb);
+  a.logTB = log_tb;
+  // exact while e * divisor < 2^32: e < W * TB (or out_count * TB) <= 2^16 * 64
+  auto magic = [](int64_t d) -> uint32_t { // 0 for d == 1 (see divide())
+    return d > 1 ? static_cast<uint32_t>(0xffffffffu / static_cast<uint32_t>(d) + 1) : 0u;
+  };
+  a.magic_W = magic(W);
+  a.magic_out = magic(output_count);
   int64_t const blocks = (a.lines + tb - 1) / tb;
   if (blocks > INT32_MAX) {
     destroy(pl);
