@@ -357,9 +357,11 @@ int paliwa_partial_create(paliwa_partial **plan, int32_t dtype, int32_t ndim,
   a.out_slot = static_cast<int32_t>(slot_of(output_start));
   a.out_local = static_cast<int32_t>(out_local);
   a.out_count = static_cast<int32_t>(output_count);
-  // lines per workgroup: as many windows as fit in 32 KiB, at most 64, a
-  // power of two (so that it divides the workgroup size)
-  int64_t tb = std::max<int64_t>(1, (32 * 1024) / (W * elem));
+  // lines per workgroup: as many windows as fit in 16 KiB, at most 64, a
+  // power of two (so that it divides the workgroup size); smaller tiles keep
+  // more workgroups per CU to cover the per-pass barriers (one slab of a
+  // 256^3 biorthogonal sweep: 16 KiB 0.070 ms, 32 KiB 0.082, 64 KiB 0.118)
+  int64_t tb = std::max<int64_t>(1, (16 * 1024) / (W * elem));
   tb = std::min<int64_t>({tb, 64, a.lines});
   int log_tb = 0;
   while ((int64_t(2) << log_tb) <= tb)
