@@ -256,7 +256,8 @@ int paliwa_partial_destroy(paliwa_partial *plan);
  * 13-23): the `bytes` bytes of a device array (a grid's elements, row-major)
  * to `filename`, headerless, native byte order -- the format the reference's
  * postprocess.py reads with numpy.fromfile.  Synchronizes `stream` (the copy
- * goes through a host buffer of at most 64 MiB). */
+ * goes through two 16 MiB pinned host buffers, allocated on the first call
+ * and kept until the process ends; calls are serialized). */
 int paliwa_dump_to_binary_file(const void *data, int64_t bytes,
                                const char *filename, void *stream);
 /* The inverse: a file of exactly `bytes` bytes into device memory

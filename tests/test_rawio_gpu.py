@@ -1,5 +1,5 @@
 """Raw dumps from device memory (paliwa_dump_to_binary_file) read with numpy,
-and loaded back, including a grid larger than the 64 MiB staging buffer."""
+and loaded back, including a grid spanning several 16 MiB staging chunks."""
 import numpy as np
 import pytest
 import torch
