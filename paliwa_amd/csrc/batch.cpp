@@ -346,7 +346,10 @@ int paliwa_batch_create(paliwa_batch **plan, int32_t dtype, int32_t ndim,
     }
   }
   // Lanes for the groups of each axis: largest group first onto the least
-  // loaded lane (work estimated by the points swept).
+  // loaded lane.  Work estimate in swept points: the group's points plus a
+  // fixed cost per launch (about 10 us of launch, fill and drain, the time the
+  // sweep needs for ~3.75 M float64 points), four times that for a long-line
+  // group, whose few workgroups make it latency-bound.
   constexpr int kLanes = 4; // 3: 1.57 ms, 8: 1.61 ms, 4: 1.43 ms on the 1325-grid set
   b->lane_of.assign(b->groups.size(), 0);
   bool concurrent = false;
@@ -365,6 +368,8 @@ int paliwa_batch_create(paliwa_batch **plan, int32_t dtype, int32_t ndim,
             e *= b->extents_of(g)[d];
           pts += e;
         }
+        int64_t const launch = dtype == PALIWA_FLOAT64 ? 3750000 : 7500000;
+        pts += b->groups[k].lds_lines ? 4 * launch : launch;
         work.push_back({pts, k});
       }
       std::sort(work.begin(), work.end(), [](auto const &x, auto const &y) {
