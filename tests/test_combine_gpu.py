@@ -93,10 +93,10 @@ def test_subspace_rules_and_float32(pw, min_count):
         comb.close()
 
 
-def test_one_and_five_dimensions(pw):
+def test_one_five_and_eight_dimensions(pw):
     from paliwa_amd.combine import SparseGridCombination
 
-    for dim, n in ((1, 1), (1, 9), (5, 8)):
+    for dim, n in ((1, 1), (1, 9), (5, 8), (8, 10)):
         levels, coefs = standard_set(dim, n, minimum=1 if n > 1 else 0)
         M = [max(l[d] for l in levels) for d in range(dim)]
         rng = np.random.default_rng(dim)
