@@ -399,6 +399,15 @@ def combine_benchmark(torch, levels, hierarchized, timed, reps):
     }
 
 
+def network_rate(run):
+    flop = 2624 * run["network_evals"]
+    tflops = flop / (run["forward_ms"] * 1e-3) / 1e12
+    return {"evaluations": run["network_evals"], "flop_per_evaluation": 2624,
+            "all_levels_forward_ms": run["forward_ms"], "TFLOPs": round(tflops, 1),
+            "bf16_dense_peak_TFLOPs": 2500.0, "frac": round(tflops / 2500.0, 4),
+            "bound": "valu"}
+
+
 def learned_benchmark():
     """The learned-lifting workload: 513^3 float32 test field (seed 4),
     training on 32 fields of 129^3 (seeds 100-131), 2^22 pairs; the learned
@@ -422,6 +431,11 @@ def learned_benchmark():
         "inverse_ms": le["inverse_ms"],
         "interpolet_forward_ms": li["forward_ms"],
         "forward_Gpoints_per_s": round(pts / (le["forward_ms"] * 1e-3) / 1e9, 2),
+        # the network at every level >= 3 of every axis: its arithmetic against
+        # the bf16 dense MFMA peak (2 * (8*32 + 32*32 + 32) = 2624 flop per
+        # evaluation); the sweep is bound by the VALU (two quarter-rate
+        # transcendentals per hidden unit in the GELU), not by the matrix cores
+        "network": network_rate(r["learned_without_fallback"]),
         "train": {k: r["train"][k] for k in ("pairs", "steps", "batch", "loss_before",
                                              "loss_after", "seconds")},
     }

@@ -339,9 +339,16 @@ def workload(level=(9, 9, 9), train_level=(7, 7, 7), train_seeds=range(100, 132)
     # the learned run with the MLP at every level >= 3 (no fallback), for reference
     all_on = [sum(1 << l for l in range(3, level[d] + 1)) for d in range(D)]
     c = q.clone()
-    forward(c, level, lmin, weights, all_on, update, overflow=overflow, features=features)
+    ms_all = timed(lambda: forward(c, level, lmin, weights, all_on, update, overflow=overflow,
+                                   features=features))
     frac, l1 = detail_stats(c.double() * scale, level, lmin, thr)
-    out["learned_without_fallback"] = {"frac_above_1e-4": frac, "detail_l1": l1}
+    # network evaluations of that run: per axis, lines x odd points of levels >= 3
+    evals = 0
+    for d in range(D):
+        lines = q.numel() // q.shape[d]
+        evals += lines * sum(1 << (l - 1) for l in range(max(3, lmin[d] + 1), level[d] + 1))
+    out["learned_without_fallback"] = {"frac_above_1e-4": frac, "detail_l1": l1,
+                                       "forward_ms": round(ms_all, 3), "network_evals": evals}
     del c
     # the plain float transform of the library (reference arithmetic), hat
     g = f.clone()
