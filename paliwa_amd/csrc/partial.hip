@@ -18,6 +18,7 @@
 // bit-identical to it), and writes back the output points only.  Ghosts and
 // the other local points are not modified.
 #include "common.hpp"
+#include "fused_sweep.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -175,6 +176,136 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// Along a strided axis whose period the fused kernel holds (a ladder of
+// consecutive levels): the whole period of each line in registers, as in the
+// fused sweep.  Rows of the window come from the local block or the ghosts,
+// every other row is zero; the ladder then runs on all rows, and only the
+// outputs are stored.  Bit-identical on the outputs: the planner puts every
+// output's dependency cone inside the window, and no point computed from the
+// zero rows is read by a point of that cone.
+struct FusedPartialArgs {
+  int32_t const *row_source; // [P] per row: local index, -(ghost + 1), kNoSource
+  int64_t lines, B, m_local, m_ghost;
+  int32_t out_row, out_local, out_count;
+  int32_t lo, hi;            // ladder: log2 storage strides lo .. hi
+  double coef[2][3];
+};
+
+template <typename T, int K, int G, int PAT, bool ASC>
+__global__ __launch_bounds__(fused::threads_for(true, G)) void partial_fused(
+    FusedPartialArgs a, T *__restrict__ local, T const *__restrict__ ghosts) {
+  using Geom = fused::TileGeom<K, G, true>;
+  constexpr int TB = Geom::TB;
+  constexpr int NT = Geom::NT;
+  constexpr int P = K * G;
+  constexpr int RS = NT / TB; // rows per staging step
+  extern __shared__ unsigned char smem[];
+  T *const lds = reinterpret_cast<T *>(smem);
+  int const tid = threadIdx.x;
+  int const c = tid % TB;
+  int const r0 = tid / TB;
+  int64_t const line = int64_t(blockIdx.x) * TB + c;
+  bool const valid = line < a.lines;
+  int64_t const ai = valid ? line / a.B : 0;
+  int64_t const bi = valid ? line - ai * a.B : 0;
+  T *const lbase = local + ai * a.m_local * a.B + bi;
+  T const *const gbase = ghosts + ai * a.m_ghost * a.B + bi;
+  for (int r = r0; r < P; r += RS) {
+    int32_t const src = a.row_source[r];
+    T v = T(0);
+    if (valid) {
+      if (src >= 0)
+        v = lbase[src * a.B];
+      else if (src != kNoSource)
+        v = gbase[int64_t(-src - 1) * a.B];
+    }
+    lds[fused::lds_index<K, G, true>(c, r, 0)] = v;
+  }
+  __syncthreads();
+  int const li = tid / G;
+  int const gi = tid % G;
+  int const lane_base = (tid & 63) - gi;
+  fused::Coefs<T> cf;
+#pragma unroll
+  for (int f = 0; f < 2; ++f)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      cf.c[f][k] = static_cast<T>(a.coef[f][k]);
+  T v[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+    v[j] = lds[fused::lds_index<K, G, true>(li, gi * K + j, 0)];
+  fused::run_ladder<T, K, G, PAT, ASC>(v, lane_base, gi, a.lo, a.hi, cf);
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+    lds[fused::lds_index<K, G, true>(li, gi * K + j, 0)] = v[j];
+  __syncthreads();
+  if (!valid)
+    return;
+  for (int i = r0; i < a.out_count; i += RS) {
+    int x = a.out_row + i;
+    if (x >= P)
+      x -= P;
+    int64_t j = a.out_local + i; // wraps only when the local run is the whole period
+    if (j >= a.m_local)
+      j -= a.m_local;
+    lbase[j * a.B] = lds[fused::lds_index<K, G, true>(c, x, 0)];
+  }
+}
+
+template <typename T, int K, int G, int PAT, bool ASC>
+hipError_t launch_partial_fused_kg(FusedPartialArgs const &a, T *local, T const *ghosts,
+                                   hipStream_t s) {
+  using Geom = fused::TileGeom<K, G, true>;
+  // the whole period of TB lines, row-major, one pad word per chunk of K rows
+  // (the fused sweep's strided layout; staged here for every G, G = 1 included)
+  size_t const bytes = sizeof(T) * size_t(K * G * Geom::TB + G + 2);
+  auto kernel = partial_fused<T, K, G, PAT, ASC>;
+  if (bytes > 65536) {
+    hipError_t const err = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               static_cast<int>(bytes));
+    if (err != hipSuccess)
+      return err;
+  }
+  int64_t const blocks = (a.lines + Geom::TB - 1) / Geom::TB;
+  hipLaunchKernelGGL(kernel, dim3(static_cast<uint32_t>(blocks)), dim3(Geom::NT), bytes, s, a,
+                     local, ghosts);
+  return hipGetLastError();
+}
+
+template <typename T, int K, int G>
+hipError_t launch_partial_fused_pat(FusedPartialArgs const &a, int pattern, bool asc,
+                                    T *local, T const *ghosts, hipStream_t s) {
+  switch (pattern) {
+  case kPatOdd:
+    return asc ? launch_partial_fused_kg<T, K, G, kPatOdd, true>(a, local, ghosts, s)
+               : launch_partial_fused_kg<T, K, G, kPatOdd, false>(a, local, ghosts, s);
+  case kPatOddEven:
+    return asc ? launch_partial_fused_kg<T, K, G, kPatOddEven, true>(a, local, ghosts, s)
+               : launch_partial_fused_kg<T, K, G, kPatOddEven, false>(a, local, ghosts, s);
+  case kPatEvenOdd:
+    return asc ? launch_partial_fused_kg<T, K, G, kPatEvenOdd, true>(a, local, ghosts, s)
+               : launch_partial_fused_kg<T, K, G, kPatEvenOdd, false>(a, local, ghosts, s);
+  default:
+    return hipErrorInvalidValue;
+  }
+}
+
+template <typename T, int L, int LMAX>
+hipError_t launch_partial_fused(FusedPartialArgs const &a, int period_log2, int pattern,
+                                bool asc, T *local, T const *ghosts, hipStream_t s) {
+  if constexpr (L > LMAX) {
+    return hipErrorInvalidValue;
+  } else {
+    if (period_log2 == L) {
+      constexpr int K = fused_chunk_of(sizeof(T), L);
+      return launch_partial_fused_pat<T, K, (1 << L) / K>(a, pattern, asc, local, ghosts, s);
+    }
+    return launch_partial_fused<T, L + 1, LMAX>(a, period_log2, pattern, asc, local, ghosts, s);
+  }
+}
+
 } // namespace
 
 struct paliwa_partial {
@@ -183,6 +314,12 @@ struct paliwa_partial {
   int32_t device = 0;
   int32_t dim = 0;
   int32_t blocks = 0;
+  // the whole-period register form (strided axis, ladder, period within the
+  // fused kernel's): used instead of partial_lines when set
+  bool fused = false;
+  int32_t level = 0, pattern = 0;
+  bool asc = false;
+  FusedPartialArgs fargs{};
   int64_t lds_bytes = 0;
   std::vector<int64_t> ghost_index;
   PartialArgs args{};
@@ -382,14 +519,57 @@ int paliwa_partial_create(paliwa_partial **plan, int32_t dtype, int32_t ndim,
   }
   pl->blocks = static_cast<int32_t>(blocks);
   pl->lds_bytes = W * tb * elem;
+  // the whole-period register form: a strided axis, a ladder of consecutive
+  // levels with a table pattern, a period the fused kernel holds, and a
+  // window of at least half the period (it computes every row of the period;
+  // on one slab of a 256^3 grid: window = period 0.058 ms against 0.070 for
+  // partial_lines, but a window of 77 rows 0.039 against 0.024)
+  std::vector<int32_t> row_source;
+  if (B > 1 && level >= 1 && nlevels >= 1 && P <= fused_max_period(dtype) && 2 * W >= P) {
+    SweepParams sp{};
+    sp.nlevels = nlevels;
+    for (int32_t i = 0; i < nlevels; ++i)
+      sp.shift[i] = static_cast<int8_t>(level - levels[i]);
+    set_filters(sp, filters, nfilters);
+    classify(sp);
+    if (sp.schedule != kScheduleGeneric && sp.pattern != kPatNone) {
+      pl->fused = true;
+      pl->level = level;
+      pl->pattern = sp.pattern;
+      pl->asc = sp.schedule == kScheduleLadderAsc;
+      FusedPartialArgs &f = pl->fargs;
+      f.lines = a.lines;
+      f.B = B;
+      f.m_local = m_local;
+      f.m_ghost = a.m_ghost;
+      f.out_row = static_cast<int32_t>(output_start);
+      f.out_local = static_cast<int32_t>(out_local);
+      f.out_count = static_cast<int32_t>(output_count);
+      f.lo = sp.lo;
+      f.hi = sp.hi;
+      for (int q = 0; q < 2; ++q)
+        for (int j = 0; j < 3; ++j)
+          f.coef[q][j] = q < sp.npass ? sp.coef[q][j] : 0.0;
+      row_source.assign(static_cast<size_t>(P), kNoSource);
+      for (int64_t x = 0; x < P; ++x) {
+        int64_t const slot = slot_of(x);
+        if (slot < W)
+          row_source[size_t(x)] = source[size_t(slot)];
+      }
+    }
+  }
   // device tables
   size_t const src_bytes = source.size() * sizeof(int32_t);
   size_t const pass_at = (src_bytes + 255) / 256 * 256;
-  size_t const total = pass_at + std::max<size_t>(1, passes.size()) * sizeof(PartialPass);
+  size_t const pass_bytes = std::max<size_t>(1, passes.size()) * sizeof(PartialPass);
+  size_t const rows_at = (pass_at + pass_bytes + 255) / 256 * 256;
+  size_t const total = rows_at + std::max<size_t>(1, row_source.size()) * sizeof(int32_t);
   std::vector<char> buf(total, 0);
   std::memcpy(buf.data(), source.data(), src_bytes);
   if (!passes.empty())
     std::memcpy(buf.data() + pass_at, passes.data(), passes.size() * sizeof(PartialPass));
+  if (!row_source.empty())
+    std::memcpy(buf.data() + rows_at, row_source.data(), row_source.size() * sizeof(int32_t));
   if (hipMalloc(&pl->dev, total) != hipSuccess ||
       hipMemcpy(pl->dev, buf.data(), total, hipMemcpyHostToDevice) != hipSuccess) {
     if (pl->dev == nullptr)
@@ -398,6 +578,7 @@ int paliwa_partial_create(paliwa_partial **plan, int32_t dtype, int32_t ndim,
     return fail(PALIWA_ERR_HIP, "partial: device tables could not be allocated");
   }
   a.slot_source = static_cast<int32_t const *>(pl->dev);
+  pl->fargs.row_source = reinterpret_cast<int32_t const *>(static_cast<char *>(pl->dev) + rows_at);
   a.passes = reinterpret_cast<PartialPass const *>(static_cast<char *>(pl->dev) + pass_at);
   *plan = pl;
   return PALIWA_OK;
@@ -422,6 +603,19 @@ int paliwa_partial_transform(const paliwa_partial *plan, void *local,
   if (local == nullptr || (ghosts == nullptr && !plan->ghost_index.empty()))
     return fail(PALIWA_ERR_INVALID_ARGUMENT, "local (and ghosts, if the plan has any) are required");
   auto s = static_cast<hipStream_t>(stream);
+  if (plan->fused) {
+    hipError_t const err =
+        plan->dtype == PALIWA_FLOAT64
+            ? launch_partial_fused<double, 1, 11>(plan->fargs, plan->level, plan->pattern,
+                                                  plan->asc, static_cast<double *>(local),
+                                                  static_cast<double const *>(ghosts), s)
+            : launch_partial_fused<float, 1, 12>(plan->fargs, plan->level, plan->pattern,
+                                                 plan->asc, static_cast<float *>(local),
+                                                 static_cast<float const *>(ghosts), s);
+    if (err != hipSuccess)
+      return fail(PALIWA_ERR_HIP, "partial_fused launch failed");
+    return PALIWA_OK;
+  }
   size_t const lds = static_cast<size_t>(plan->lds_bytes);
   if (plan->dtype == PALIWA_FLOAT64)
     partial_lines<double><<<plan->blocks, kThreads, lds, s>>>(
