@@ -28,12 +28,18 @@
 #include <new>
 #include <vector>
 
+// The whole-period register form runs when SHARE * window >= period.
+#ifndef PALIWA_PARTIAL_FUSED_SHARE
+#define PALIWA_PARTIAL_FUSED_SHARE 2
+#endif
+
 namespace {
 
 using namespace paliwa_hip;
 
 constexpr int kThreads = 256;
 constexpr int32_t kNoSource = std::numeric_limits<int32_t>::min();
+constexpr int16_t kNoRow = std::numeric_limits<int16_t>::min();
 constexpr int64_t kMaxWindowBytes = 64 * 1024;
 
 struct PartialPass {
@@ -184,72 +190,115 @@ __global__ void __launch_bounds__(kThreads)
 // output's dependency cone inside the window, and no point computed from the
 // zero rows is read by a point of that cone.
 struct FusedPartialArgs {
-  int32_t const *row_source; // [P] per row: local index, -(ghost + 1), kNoSource
+  int16_t const *row_source; // [P] per row: local index, -(ghost + 1), kNoRow
   int64_t lines, B, m_local, m_ghost;
   int32_t out_row, out_local, out_count;
   int32_t lo, hi;            // ladder: log2 storage strides lo .. hi
   double coef[2][3];
 };
 
+// LDS of partial_fused: the whole period of TB lines, row-major with one pad
+// word per chunk of K rows (the fused sweep's strided layout, staged here for
+// every G, G = 1 included), then the row table (int16 per row).
+template <typename T, int K, int G>
+constexpr size_t fused_partial_table_at() {
+  return (sizeof(T) * size_t(K * G * fused::TileGeom<K, G, true>::TB + G + 2) + 15) / 16 * 16;
+}
+
 template <typename T, int K, int G, int PAT, bool ASC>
 __global__ __launch_bounds__(fused::threads_for(true, G)) void partial_fused(
-    FusedPartialArgs a, T *__restrict__ local, T const *__restrict__ ghosts) {
+    FusedPartialArgs a, T *__restrict__ local, T const *__restrict__ ghosts,
+    int64_t ntiles) {
   using Geom = fused::TileGeom<K, G, true>;
   constexpr int TB = Geom::TB;
   constexpr int NT = Geom::NT;
   constexpr int P = K * G;
-  constexpr int RS = NT / TB; // rows per staging step
+  constexpr int RS = NT / TB; // rows per staging step (= G): K rows per thread
   extern __shared__ unsigned char smem[];
   T *const lds = reinterpret_cast<T *>(smem);
   int const tid = threadIdx.x;
   int const c = tid % TB;
   int const r0 = tid / TB;
-  int64_t const line = int64_t(blockIdx.x) * TB + c;
-  bool const valid = line < a.lines;
-  int64_t const ai = valid ? line / a.B : 0;
-  int64_t const bi = valid ? line - ai * a.B : 0;
-  T *const lbase = local + ai * a.m_local * a.B + bi;
-  T const *const gbase = ghosts + ai * a.m_ghost * a.B + bi;
-  for (int r = r0; r < P; r += RS) {
-    int32_t const src = a.row_source[r];
-    T v = T(0);
-    if (valid) {
-      if (src >= 0)
-        v = lbase[src * a.B];
-      else if (src != kNoSource)
-        v = gbase[int64_t(-src - 1) * a.B];
-    }
-    lds[fused::lds_index<K, G, true>(c, r, 0)] = v;
-  }
+  uint32_t const nwg = gridDim.x;
+  // the row of the block or of the ghost array behind row r0 + k * RS, at
+  // [r0 * K + k] after the tile: a thread's K entries are adjacent
+  int16_t *const table = reinterpret_cast<int16_t *>(smem + fused_partial_table_at<T, K, G>());
+  for (int x = tid; x < P; x += NT)
+    table[(x % RS) * K + x / RS] = a.row_source[x];
   __syncthreads();
-  int const li = tid / G;
-  int const gi = tid % G;
-  int const lane_base = (tid & 63) - gi;
+  // this thread's column of a tile in the block and in the ghost array (the
+  // ghost array has the block's shape with m_ghost rows); null: past the grid
+  struct Column {
+    T *l;
+    T const *g;
+  };
+  auto column = [&](int64_t tile) -> Column {
+    int64_t const line = tile * TB + c;
+    if (tile >= ntiles || line >= a.lines)
+      return {nullptr, nullptr};
+    int64_t const ai = line / a.B;
+    int64_t const bi = line - ai * a.B;
+    return {local + ai * a.m_local * a.B + bi,
+            ghosts == nullptr ? nullptr : ghosts + ai * a.m_ghost * a.B + bi};
+  };
+  // rows r0, r0 + RS, ... of the column into registers (zeros off the window)
+  auto fetch = [&](Column col, T (&buf)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      int const src = table[r0 * K + k];
+      T const *const base = src >= 0 ? col.l : col.g;
+      int64_t const row = src >= 0 ? src : -src - 1;
+      T v = T(0);
+      if (col.l != nullptr && src != kNoRow)
+        v = base[row * a.B];
+      buf[k] = v;
+    }
+  };
   fused::Coefs<T> cf;
 #pragma unroll
   for (int f = 0; f < 2; ++f)
 #pragma unroll
     for (int k = 0; k < 3; ++k)
       cf.c[f][k] = static_cast<T>(a.coef[f][k]);
-  T v[K];
+  int const li = tid / G;
+  int const gi = tid % G;
+  int const lane_base = (tid & 63) - gi;
+
+  int64_t t = fused::xcd_tile(blockIdx.x, nwg);
+  T buf[K];
+  Column cur = column(t);
+  fetch(cur, buf);
+  for (; t < ntiles; t += nwg) {
+    __syncthreads(); // the previous tile's output reads are done
 #pragma unroll
-  for (int j = 0; j < K; ++j)
-    v[j] = lds[fused::lds_index<K, G, true>(li, gi * K + j, 0)];
-  fused::run_ladder<T, K, G, PAT, ASC>(v, lane_base, gi, a.lo, a.hi, cf);
+    for (int k = 0; k < K; ++k)
+      lds[fused::lds_index<K, G, true>(c, r0 + k * RS, 0)] = buf[k];
+    __syncthreads();
+    // the next tile's loads stay in flight while this one computes
+    Column const nxt = column(t + nwg);
+    if (t + nwg < ntiles)
+      fetch(nxt, buf);
+    T v[K];
 #pragma unroll
-  for (int j = 0; j < K; ++j)
-    lds[fused::lds_index<K, G, true>(li, gi * K + j, 0)] = v[j];
-  __syncthreads();
-  if (!valid)
-    return;
-  for (int i = r0; i < a.out_count; i += RS) {
-    int x = a.out_row + i;
-    if (x >= P)
-      x -= P;
-    int64_t j = a.out_local + i; // wraps only when the local run is the whole period
-    if (j >= a.m_local)
-      j -= a.m_local;
-    lbase[j * a.B] = lds[fused::lds_index<K, G, true>(c, x, 0)];
+    for (int j = 0; j < K; ++j)
+      v[j] = lds[fused::lds_index<K, G, true>(li, gi * K + j, 0)];
+    fused::run_ladder<T, K, G, PAT, ASC>(v, lane_base, gi, a.lo, a.hi, cf);
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      lds[fused::lds_index<K, G, true>(li, gi * K + j, 0)] = v[j];
+    __syncthreads();
+    if (cur.l != nullptr) {
+      for (int i = r0; i < a.out_count; i += RS) {
+        int x = a.out_row + i;
+        if (x >= P)
+          x -= P;
+        int64_t j = a.out_local + i; // wraps only when the local run is the whole period
+        if (j >= a.m_local)
+          j -= a.m_local;
+        cur.l[j * a.B] = lds[fused::lds_index<K, G, true>(c, x, 0)];
+      }
+    }
+    cur = nxt;
   }
 }
 
@@ -257,9 +306,7 @@ template <typename T, int K, int G, int PAT, bool ASC>
 hipError_t launch_partial_fused_kg(FusedPartialArgs const &a, T *local, T const *ghosts,
                                    hipStream_t s) {
   using Geom = fused::TileGeom<K, G, true>;
-  // the whole period of TB lines, row-major, one pad word per chunk of K rows
-  // (the fused sweep's strided layout; staged here for every G, G = 1 included)
-  size_t const bytes = sizeof(T) * size_t(K * G * Geom::TB + G + 2);
+  size_t const bytes = fused_partial_table_at<T, K, G>() + sizeof(int16_t) * K * G;
   auto kernel = partial_fused<T, K, G, PAT, ASC>;
   if (bytes > 65536) {
     hipError_t const err = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
@@ -268,9 +315,11 @@ hipError_t launch_partial_fused_kg(FusedPartialArgs const &a, T *local, T const 
     if (err != hipSuccess)
       return err;
   }
-  int64_t const blocks = (a.lines + Geom::TB - 1) / Geom::TB;
-  hipLaunchKernelGGL(kernel, dim3(static_cast<uint32_t>(blocks)), dim3(Geom::NT), bytes, s, a,
-                     local, ghosts);
+  int64_t const tiles = (a.lines + Geom::TB - 1) / Geom::TB;
+  int64_t const resident = fused::resident_workgroups(reinterpret_cast<const void *>(kernel),
+                                                      bytes, Geom::NT);
+  uint32_t const grid = static_cast<uint32_t>(std::min<int64_t>(tiles, resident));
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(Geom::NT), bytes, s, a, local, ghosts, tiles);
   return hipGetLastError();
 }
 
@@ -522,10 +571,12 @@ int paliwa_partial_create(paliwa_partial **plan, int32_t dtype, int32_t ndim,
   // the whole-period register form: a strided axis, a ladder of consecutive
   // levels with a table pattern, a period the fused kernel holds, and a
   // window of at least half the period (it computes every row of the period;
-  // on one slab of a 256^3 grid: window = period 0.058 ms against 0.070 for
-  // partial_lines, but a window of 77 rows 0.039 against 0.024)
-  std::vector<int32_t> row_source;
-  if (B > 1 && level >= 1 && nlevels >= 1 && P <= fused_max_period(dtype) && 2 * W >= P) {
+  // on one slab of a 256^3 float64 grid: window = period 0.035 ms against
+  // 0.070 for partial_lines, a window of 77 rows 0.025 against 0.024, of 33
+  // rows 0.024 against 0.011)
+  std::vector<int16_t> row_source;
+  if (B > 1 && level >= 1 && nlevels >= 1 && P <= fused_max_period(dtype) &&
+      PALIWA_PARTIAL_FUSED_SHARE * W >= P) {
     SweepParams sp{};
     sp.nlevels = nlevels;
     for (int32_t i = 0; i < nlevels; ++i)
@@ -550,11 +601,12 @@ int paliwa_partial_create(paliwa_partial **plan, int32_t dtype, int32_t ndim,
       for (int q = 0; q < 2; ++q)
         for (int j = 0; j < 3; ++j)
           f.coef[q][j] = q < sp.npass ? sp.coef[q][j] : 0.0;
-      row_source.assign(static_cast<size_t>(P), kNoSource);
+      // local and ghost indices are below the period (at most 2^12 here)
+      row_source.assign(static_cast<size_t>(P), kNoRow);
       for (int64_t x = 0; x < P; ++x) {
         int64_t const slot = slot_of(x);
-        if (slot < W)
-          row_source[size_t(x)] = source[size_t(slot)];
+        if (slot < W && source[size_t(slot)] != kNoSource)
+          row_source[size_t(x)] = static_cast<int16_t>(source[size_t(slot)]);
       }
     }
   }
@@ -563,13 +615,13 @@ int paliwa_partial_create(paliwa_partial **plan, int32_t dtype, int32_t ndim,
   size_t const pass_at = (src_bytes + 255) / 256 * 256;
   size_t const pass_bytes = std::max<size_t>(1, passes.size()) * sizeof(PartialPass);
   size_t const rows_at = (pass_at + pass_bytes + 255) / 256 * 256;
-  size_t const total = rows_at + std::max<size_t>(1, row_source.size()) * sizeof(int32_t);
+  size_t const total = rows_at + std::max<size_t>(1, row_source.size()) * sizeof(int16_t);
   std::vector<char> buf(total, 0);
   std::memcpy(buf.data(), source.data(), src_bytes);
   if (!passes.empty())
     std::memcpy(buf.data() + pass_at, passes.data(), passes.size() * sizeof(PartialPass));
   if (!row_source.empty())
-    std::memcpy(buf.data() + rows_at, row_source.data(), row_source.size() * sizeof(int32_t));
+    std::memcpy(buf.data() + rows_at, row_source.data(), row_source.size() * sizeof(int16_t));
   if (hipMalloc(&pl->dev, total) != hipSuccess ||
       hipMemcpy(pl->dev, buf.data(), total, hipMemcpyHostToDevice) != hipSuccess) {
     if (pl->dev == nullptr)
@@ -578,7 +630,7 @@ int paliwa_partial_create(paliwa_partial **plan, int32_t dtype, int32_t ndim,
     return fail(PALIWA_ERR_HIP, "partial: device tables could not be allocated");
   }
   a.slot_source = static_cast<int32_t const *>(pl->dev);
-  pl->fargs.row_source = reinterpret_cast<int32_t const *>(static_cast<char *>(pl->dev) + rows_at);
+  pl->fargs.row_source = reinterpret_cast<int16_t const *>(static_cast<char *>(pl->dev) + rows_at);
   a.passes = reinterpret_cast<PartialPass const *>(static_cast<char *>(pl->dev) + pass_at);
   *plan = pl;
   return PALIWA_OK;

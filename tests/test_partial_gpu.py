@@ -91,6 +91,32 @@ def test_blocks_with_ghosts(pp, shape_levels, dim):
                         assert np.array_equal(got[rest], block[rest])
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_many_tiles_per_workgroup(pp, dtype):
+    """Slabs of a 256^3 grid along axis 0 (65536 lines, several tiles per
+    resident workgroup, so the next tile's rows are fetched while the current
+    one computes): the whole-period ladder (biorthogonal down to level 0, a
+    window of the whole period) and a short window (hat), slabs with and
+    without wrap."""
+    level, dim, L, P = [8, 8, 8], 0, 8, 256
+    rng = np.random.default_rng(41)
+    full = rng.standard_normal([P, P, P]).astype(dtype)
+    tdtype = torch.float64 if dtype == np.float64 else torch.float32
+    for name, lmin in (("biorthogonal", 0), ("hat", 0)):
+        filters = ob.FORWARD[name]
+        levels = list(range(L, lmin, -1))
+        want = full.copy()
+        ob.transform_in(want, level, dim, levels, filters)
+        for start, count, out_off, out_count in ((0, 32, 0, 32), (240, 40, 5, 30)):
+            block, got, run = _run(pp, full, level, dim, levels, filters, start, count,
+                                   out_off, out_count, tdtype)
+            exp = np.take(want, run, axis=dim)
+            assert np.array_equal(got[out_off:out_off + out_count],
+                                  exp[out_off:out_off + out_count]), (name, start, dtype)
+            assert np.array_equal(got[:out_off], block[:out_off])
+            assert np.array_equal(got[out_off + out_count:], block[out_off + out_count:])
+
+
 def test_single_allocation_form(pp):
     """The whole axis held locally: no ghosts; outputs anywhere, wrapping."""
     level, dim = [6, 4], 0
