@@ -108,6 +108,27 @@ def test_multidimensional_bit_exact(pw, name):
             assert_same(back, oracle_named(fwd, level, lmin, name, False), f"{extents} inv")
 
 
+def test_maximum_dimension_count(pw):
+    """26 axes (the C ABI's limit): level-0 axes of one or two stored points,
+    level-1 axes with and without the periodic image, one longer axis; every
+    axis sweeps a different [A][m][B] view."""
+    rng = np.random.default_rng(16)
+    spec = [(1, 2), (0, 1), (1, 3), (0, 1), (0, 2), (1, 2), (2, 5), (0, 1), (0, 1), (1, 2),
+            (1, 2), (0, 1), (1, 3), (0, 1), (0, 1), (1, 2), (0, 2), (0, 1), (1, 3), (1, 2),
+            (0, 1), (1, 2), (0, 1), (0, 1), (0, 1), (3, 8)]  # 552,960 points
+    assert len(spec) == 26
+    level = [l for l, _ in spec]
+    extents = [e for _, e in spec]
+    for name in WAVELETS:
+        for dtype in (np.float64, np.float32):
+            a = closed_random(rng, extents, level, dtype)
+            lmin = [0] * 26
+            fwd = gpu_named(pw, a, level, lmin, name, True)
+            assert_same(fwd, oracle_named(a, level, lmin, name, True), f"26-d {name} fwd")
+            back = gpu_named(pw, fwd, level, lmin, name, False)
+            assert_same(back, oracle_named(fwd, level, lmin, name, False), f"26-d {name} inv")
+
+
 @pytest.mark.parametrize("path", ["fused", "lds_line", "per_pass"])
 def test_kernel_paths_bit_exact(pw, path):
     rng = np.random.default_rng(13)
