@@ -78,7 +78,7 @@ __device__ __forceinline__ float gelu_gate(float z) { // sigma(2a)
   // g3 = 0.044715 g1: the constants folded into one v_exp_f32 argument
   constexpr float g1 = -2.0f * 0.7978845608028654f * 1.4426950408889634f;
   constexpr float g3 = 0.044715f * g1;
-  float const e = __builtin_amdgcn_exp2f(z * (g1 + g3 * (z * z)));
+  float const e = __builtin_amdgcn_exp2f(z * __builtin_fmaf(g3, z * z, g1));
   return __builtin_amdgcn_rcpf(1.0f + e);
 }
 __device__ __forceinline__ float gelu(float z) { return z * gelu_gate(z); }
@@ -157,15 +157,18 @@ __device__ __forceinline__ void mlp16(Net const &net, bf16x4 const (&xin)[NB],
   for (int b = 0; b < NB; ++b)
     partial[b] = 0.0f;
 #pragma unroll
-  for (int blk = 0; blk < 2; ++blk)
+  for (int blk = 0; blk < 2; ++blk) {
+    // the layer-2 bias is the accumulator's initial value
+    f32x4 const bias = {net.b2[blk][0], net.b2[blk][1], net.b2[blk][2], net.b2[blk][3]};
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w2[blk][0], h1[b][0], zero, 0, 0, 0);
+      f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w2[blk][0], h1[b][0], bias, 0, 0, 0);
       d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w2[blk][1], h1[b][1], d, 0, 0, 0);
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        partial[b] += net.w3[blk][j] * round_bf16(gelu(d[j] + net.b2[blk][j]));
+        partial[b] = __builtin_fmaf(net.w3[blk][j], round_bf16(gelu(d[j])), partial[b]);
     }
+  }
   // sum over the 4 lanes holding the same point (a + b == b + a exactly, so
   // every lane ends with the same bits)
 #pragma unroll
