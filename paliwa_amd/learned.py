@@ -1,7 +1,7 @@
 """Learned lifting: an integer lifting transform whose predict step at levels
 >= 3 adds a small MLP correction to the interpolet (hat) prediction.  An
 extension on top of the plain method, which stays the fallback (per axis and
-level) and the judge of detail sparsity.  The HIP kernels are declared in
+level) and the reference point for detail sparsity.  The HIP kernels are declared in
 include/paliwa_hip.h (paliwa_learned_*); this module drives them.
 
 Pipeline:
