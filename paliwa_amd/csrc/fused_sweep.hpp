@@ -229,20 +229,25 @@ __device__ __forceinline__ void run_generic(T (&v)[K], int gbase, int gi,
   }
 }
 
-// LDS geometry of a tile of TB lines.
-//   contiguous axis: line-major, point r of line l at l*LP + r + (r >> LOGK)
-//   strided axis:    row-major,  point r of column c at r*TB + c + (r >> LOGK)
-// One pad word per chunk puts the K-strided register loads of the G lanes of
-// a line on distinct banks.
-template <int K, int G, bool STRIDED> struct TileGeom {
+// LDS geometry of a tile of TB lines of ES-byte points.
+//   contiguous axis: line-major, point r of line l at l*LP + r + (r >> LOGK)*PAD
+//   strided axis:    row-major,  point r of column c at r*TB + c + (r >> LOGK)*PAD
+// PAD points after every chunk of K rows put the register loads of a
+// wavefront's lanes (G lanes per line, chunks K rows apart) on distinct banks
+// of the 32 x 4-byte LDS banks: a b64 access serves 16 lanes per cycle, a b32
+// access 32, and lane gi of a line lands on bank offset gi * PAD * ES / 4, so
+// PAD = 128 / (ES * G) spreads the G lanes (and the lines beside them) over
+// all banks; 1 from G = 16 (float64) or 32 (float32) up.
+template <int K, int G, bool STRIDED, int ES> struct TileGeom {
   static constexpr int NT = threads_for(STRIDED, G);
   static constexpr int TB = NT / G;
   static constexpr int LOGK = ilog2(K);
-  __host__ __device__ static int line_pitch(int m) { return (m + G) | 1; }
+  static constexpr int PAD = (G == 1 || 128 / (ES * G) < 1) ? 1 : 128 / (ES * G);
+  __host__ __device__ static int line_pitch(int m) { return (m + G * PAD) | 1; }
   __host__ __device__ static int lds_elems(int m, bool strided) {
     if (strided && (G == 1 || kStridedDirect))
       return 0; // columns go straight to registers
-    return strided ? m * TB + G + 2 : TB * line_pitch(m);
+    return strided ? m * TB + G * PAD + 2 : TB * line_pitch(m);
   }
 };
 
@@ -270,7 +275,7 @@ template <typename T, int K, int G, bool STRIDED>
 __device__ __forceinline__ Stage<T> load_tile(SweepParams const &p,
                                               TileRef const &ref,
                                               T (&buf)[K + 1]) {
-  using Geom = TileGeom<K, G, STRIDED>;
+  using Geom = TileGeom<K, G, STRIDED, sizeof(T)>;
   constexpr int TB = Geom::TB;
   int const tid = threadIdx.x;
   int const m = p.m;
@@ -317,13 +322,13 @@ __device__ __forceinline__ Stage<T> load_tile(SweepParams const &p,
   return st;
 }
 
-template <int K, int G, bool STRIDED>
+template <int K, int G, bool STRIDED, int ES>
 __device__ __forceinline__ int lds_index(int line, int r, int LP) {
-  using Geom = TileGeom<K, G, STRIDED>;
+  using Geom = TileGeom<K, G, STRIDED, ES>;
   if constexpr (STRIDED)
-    return r * Geom::TB + line + (r >> Geom::LOGK);
+    return r * Geom::TB + line + (r >> Geom::LOGK) * Geom::PAD;
   else
-    return line * LP + r + (r >> Geom::LOGK);
+    return line * LP + r + (r >> Geom::LOGK) * Geom::PAD;
 }
 
 // Registers holding a loaded tile -> LDS.
@@ -332,7 +337,7 @@ __device__ __forceinline__ void stash_tile(SweepParams const &p,
                                            Stage<T> const &st,
                                            T const (&buf)[K + 1], T *lds,
                                            uint32_t magic_m, int LP) {
-  using Geom = TileGeom<K, G, STRIDED>;
+  using Geom = TileGeom<K, G, STRIDED, sizeof(T)>;
   int const tid = threadIdx.x;
   int const m = p.m;
   if constexpr (!STRIDED) {
@@ -342,7 +347,7 @@ __device__ __forceinline__ void stash_tile(SweepParams const &p,
       if ((st.full && i < K) || e < st.count) {
         int const l = __umulhi(static_cast<uint32_t>(e), magic_m);
         int const r = e - l * m;
-        lds[lds_index<K, G, false>(l, r, LP)] = buf[i];
+        lds[lds_index<K, G, false, sizeof(T)>(l, r, LP)] = buf[i];
       }
     }
   } else {
@@ -351,9 +356,9 @@ __device__ __forceinline__ void stash_tile(SweepParams const &p,
     if (st.lane != nullptr) {
 #pragma unroll
       for (int i = 0; i < K; ++i)
-        lds[lds_index<K, G, true>(col, row0 + i * G, LP)] = buf[i];
+        lds[lds_index<K, G, true, sizeof(T)>(col, row0 + i * G, LP)] = buf[i];
       if (st.image != 0)
-        lds[lds_index<K, G, true>(col, 1 << p.L, LP)] = buf[K];
+        lds[lds_index<K, G, true, sizeof(T)>(col, 1 << p.L, LP)] = buf[K];
     }
   }
 }
@@ -363,7 +368,7 @@ template <typename T, int K, int G, bool STRIDED>
 __device__ __forceinline__ void store_tile(SweepParams const &p,
                                            Stage<T> const &st, T const *lds,
                                            uint32_t magic_m, int LP) {
-  using Geom = TileGeom<K, G, STRIDED>;
+  using Geom = TileGeom<K, G, STRIDED, sizeof(T)>;
   int const tid = threadIdx.x;
   int const m = p.m;
   T out[K + 1];
@@ -374,7 +379,7 @@ __device__ __forceinline__ void store_tile(SweepParams const &p,
       if ((st.full && i < K) || e < st.count) {
         int const l = __umulhi(static_cast<uint32_t>(e), magic_m);
         int const r = e - l * m;
-        out[i] = lds[lds_index<K, G, false>(l, r, LP)];
+        out[i] = lds[lds_index<K, G, false, sizeof(T)>(l, r, LP)];
       }
     }
     T *const s = st.src + tid;
@@ -396,9 +401,9 @@ __device__ __forceinline__ void store_tile(SweepParams const &p,
     if (st.lane != nullptr) {
 #pragma unroll
       for (int i = 0; i < K; ++i)
-        out[i] = lds[lds_index<K, G, true>(col, row0 + i * G, LP)];
+        out[i] = lds[lds_index<K, G, true, sizeof(T)>(col, row0 + i * G, LP)];
       if (st.image != 0)
-        out[K] = lds[lds_index<K, G, true>(col, 1 << p.L, LP)];
+        out[K] = lds[lds_index<K, G, true, sizeof(T)>(col, 1 << p.L, LP)];
 #pragma unroll
       for (int i = 0; i < K; ++i)
         st.lane[i * st.step] = out[i];
@@ -418,7 +423,7 @@ template <typename T, int K, int G, bool STRIDED, int MODE, int PAT, bool ASC,
 __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
                                             Src const &src, uint32_t magic_m,
                                             int64_t ntiles, T *lds) {
-  using Geom = TileGeom<K, G, STRIDED>;
+  using Geom = TileGeom<K, G, STRIDED, sizeof(T)>;
   constexpr int TB = Geom::TB;
   int const tid = threadIdx.x;
   int const m = p.m;
@@ -515,7 +520,7 @@ __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
       T v[K];
 #pragma unroll
       for (int j = 0; j < K; ++j)
-        v[j] = lds[lds_index<K, G, STRIDED>(li, gi * K + j, LP)];
+        v[j] = lds[lds_index<K, G, STRIDED, sizeof(T)>(li, gi * K + j, LP)];
       if constexpr (kInt) {
         // lines past the grid's end (last tile) hold stale LDS words; they
         // are never stored, but their arithmetic must not raise the overflow
@@ -531,9 +536,9 @@ __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
       compute(v);
 #pragma unroll
       for (int j = 0; j < K; ++j)
-        lds[lds_index<K, G, STRIDED>(li, gi * K + j, LP)] = v[j];
+        lds[lds_index<K, G, STRIDED, sizeof(T)>(li, gi * K + j, LP)] = v[j];
       if (m != n && gi == 0)
-        lds[lds_index<K, G, STRIDED>(li, n, LP)] = v[0];
+        lds[lds_index<K, G, STRIDED, sizeof(T)>(li, n, LP)] = v[0];
       __syncthreads();
       store_tile<T, K, G, STRIDED>(p, cur, lds, magic_m, LP);
       cur = nxt;
@@ -604,7 +609,7 @@ hipError_t launch_one(Kernel kernel, SweepParams const &p, size_t bytes,
 
 template <typename T, int K, int G, bool STRIDED>
 hipError_t launch_kg_layout(SweepParams const &p, hipStream_t stream) {
-  using Geom = TileGeom<K, G, STRIDED>;
+  using Geom = TileGeom<K, G, STRIDED, sizeof(T)>;
   int64_t const lines = p.A * p.B;
   int64_t const tiles = (lines + Geom::TB - 1) / Geom::TB;
   size_t const bytes = sizeof(T) * Geom::lds_elems(p.m, STRIDED);
@@ -638,7 +643,7 @@ hipError_t launch_kg(SweepParams const &p, hipStream_t stream) {
 template <typename T, int K, int G, bool STRIDED>
 hipError_t launch_batched_layout(SweepParams const &p, ManyGrids grids,
                                  int64_t tiles, hipStream_t stream) {
-  using Geom = TileGeom<K, G, STRIDED>;
+  using Geom = TileGeom<K, G, STRIDED, sizeof(T)>;
   size_t const bytes = sizeof(T) * Geom::lds_elems(p.m, STRIDED);
   bool const asc = p.schedule == kScheduleLadderAsc;
   if (p.schedule != kScheduleLadderAsc && p.schedule != kScheduleLadderDesc)
