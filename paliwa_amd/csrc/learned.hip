@@ -65,6 +65,17 @@ __device__ __forceinline__ float from_bf16(short b) {
 __device__ __forceinline__ float round_bf16(float x) {
   return from_bf16(to_bf16(x));
 }
+// four fp32 -> one bf16x4 fragment, two values per v_cvt_pk_bf16_f32 (the
+// same rounding as to_bf16)
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ bf16x4 pack_bf16(f32x4 v) {
+  bf16x2_t const lo = __builtin_convertvector(f32x2_t{v[0], v[1]}, bf16x2_t);
+  bf16x2_t const hi = __builtin_convertvector(f32x2_t{v[2], v[3]}, bf16x2_t);
+  u32x2_t const w = {__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi)};
+  return __builtin_bit_cast(bf16x4, w);
+}
 
 // GELU, tanh form, written as the identity 0.5 z (1 + tanh a) = z sigma(2a)
 // with a = sqrt(2/pi) (z + 0.044715 z^3): one hardware exponential and one
@@ -96,7 +107,7 @@ struct Net {
   bf16x4 w1[2];    // A fragments of [W1 | b1 | 0] (K padded 8 -> 16), 2 row blocks
   bf16x4 w2[2][2]; // A fragments of W2, [row block][k block]
   float b2[2][4];  // bias of the 4 units this lane holds per row block
-  float w3[2][4];  // output weights of those units (bf16 values)
+  bf16x4 w3[2];    // A fragments of the output weights, the same in every row, [k block]
   float b3;
 };
 
@@ -126,7 +137,7 @@ __device__ __forceinline__ void load_net(float const *prm, Net &net) {
     for (int j = 0; j < 4; ++j) {
       int const u = 16 * blk + kq + j; // D-layout unit of this lane
       net.b2[blk][j] = prm[kB2 + u];
-      net.w3[blk][j] = round_bf16(prm[kW3 + u]);
+      net.w3[blk][j] = to_bf16(prm[kW3 + u]); // k = unit u of k block blk
     }
   }
   net.b3 = prm[kB3];
@@ -148,14 +159,12 @@ __device__ __forceinline__ void mlp16(Net const &net, bf16x4 const (&xin)[NB],
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       f32x4 const d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w1[blk], xin[b], zero, 0, 0, 0);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        h1[b][blk][j] = to_bf16(gelu(d[j]));
+      h1[b][blk] = pack_bf16(f32x4{gelu(d[0]), gelu(d[1]), gelu(d[2]), gelu(d[3])});
     }
-  float partial[NB];
-#pragma unroll
-  for (int b = 0; b < NB; ++b)
-    partial[b] = 0.0f;
+  // layer 2, then the output layer on the matrix cores: A = the output
+  // weights in every row, so every row of D (every lane) holds the output of
+  // its point; the output bias is the accumulator's initial value
+  bf16x4 h2[NB][2];
 #pragma unroll
   for (int blk = 0; blk < 2; ++blk) {
     // the layer-2 bias is the accumulator's initial value
@@ -164,19 +173,15 @@ __device__ __forceinline__ void mlp16(Net const &net, bf16x4 const (&xin)[NB],
     for (int b = 0; b < NB; ++b) {
       f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w2[blk][0], h1[b][0], bias, 0, 0, 0);
       d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w2[blk][1], h1[b][1], d, 0, 0, 0);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        partial[b] = __builtin_fmaf(net.w3[blk][j], round_bf16(gelu(d[j])), partial[b]);
+      h2[b][blk] = pack_bf16(f32x4{gelu(d[0]), gelu(d[1]), gelu(d[2]), gelu(d[3])});
     }
   }
-  // sum over the 4 lanes holding the same point (a + b == b + a exactly, so
-  // every lane ends with the same bits)
+  f32x4 const b3 = {net.b3, net.b3, net.b3, net.b3};
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    float v = partial[b];
-    v += __shfl_xor(v, 16, 64);
-    v += __shfl_xor(v, 32, 64);
-    out[b] = v + net.b3;
+    f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w3[0], h2[b][0], b3, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w3[1], h2[b][1], d, 0, 0, 0);
+    out[b] = d[0];
   }
 }
 
@@ -386,9 +391,9 @@ __global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
         if (a.weights != nullptr) {
           uint32_t word[4];
 #pragma unroll
-          for (int j = 0; j < 4; ++j)
-            word[j] = static_cast<uint16_t>(to_bf16(u[2 * j])) |
-                      (uint32_t(static_cast<uint16_t>(to_bf16(u[2 * j + 1]))) << 16);
+          for (int j = 0; j < 4; ++j) // bf16 pairs, one v_cvt_pk_bf16_f32 each
+            word[j] = __builtin_bit_cast(
+                uint32_t, __builtin_convertvector(f32x2_t{u[2 * j], u[2 * j + 1]}, bf16x2_t));
 #pragma unroll
           for (int g = 0; g < 4; ++g) {
             int const src = 16 * g + l;
@@ -399,12 +404,7 @@ __global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
             // lanes q = 0, 1: inputs 4q..4q+3; q = 2: the bias input 1; q = 3: 0
             uint32_t const lo = q == 0 ? w[0] : q == 1 ? w[2] : q == 2 ? 0x3f80u : 0u;
             uint32_t const hi = q == 0 ? w[1] : q == 1 ? w[3] : 0u;
-            bf16x4 xin;
-            xin[0] = static_cast<short>(lo & 0xffffu);
-            xin[1] = static_cast<short>(lo >> 16);
-            xin[2] = static_cast<short>(hi & 0xffffu);
-            xin[3] = static_cast<short>(hi >> 16);
-            bf16x4 const xs[1] = {xin};
+            bf16x4 const xs[1] = {__builtin_bit_cast(bf16x4, u32x2_t{lo, hi})};
             float out[1];
             mlp16<1>(net, xs, out);
             r = g == q ? out[0] : r;
