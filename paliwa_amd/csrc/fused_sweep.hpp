@@ -237,12 +237,14 @@ __device__ __forceinline__ void run_generic(T (&v)[K], int gbase, int gi,
 // of the 32 x 4-byte LDS banks: a b64 access serves 16 lanes per cycle, a b32
 // access 32, and lane gi of a line lands on bank offset gi * PAD * ES / 4, so
 // PAD = 128 / (ES * G) spreads the G lanes (and the lines beside them) over
-// all banks; 1 from G = 16 (float64) or 32 (float32) up.
+// all banks; 1 from G = 16 (float64) or 32 (float32) up.  With one lane per
+// line (G = 1) the lanes are a line pitch apart, which is odd: no pad, so that
+// the staging copy of lines with an image (odd m) runs without gaps.
 template <int K, int G, bool STRIDED, int ES> struct TileGeom {
   static constexpr int NT = threads_for(STRIDED, G);
   static constexpr int TB = NT / G;
   static constexpr int LOGK = ilog2(K);
-  static constexpr int PAD = (G == 1 || 128 / (ES * G) < 1) ? 1 : 128 / (ES * G);
+  static constexpr int PAD = G == 1 ? 0 : (128 / (ES * G) < 1 ? 1 : 128 / (ES * G));
   __host__ __device__ static int line_pitch(int m) { return (m + G * PAD) | 1; }
   __host__ __device__ static int lds_elems(int m, bool strided) {
     if (strided && (G == 1 || kStridedDirect))
