@@ -88,7 +88,9 @@ __global__ __launch_bounds__(kThreads) void lds_line_sweep(SweepParams p) {
   T *const line = reinterpret_cast<T *>(smem_raw);
   T *const g = static_cast<T *>(p.data);
   int64_t const lines = p.A * p.B;
-  for (int64_t ln = blockIdx.x; ln < lines; ln += gridDim.x) {
+  // consecutive lines (neighbouring columns of a strided axis, which share
+  // every cache line) run on one XCD and meet in its L2
+  for (int64_t ln = fused::xcd_tile(blockIdx.x, gridDim.x); ln < lines; ln += gridDim.x) {
     int64_t const a = ln / p.B;
     int64_t const b = ln - a * p.B;
     sweep_line(p, g + a * p.m * p.B + b, p.B, line);
@@ -104,7 +106,7 @@ __global__ __launch_bounds__(kThreads) void lds_line_batched(SweepParams p,
                                                              int64_t lines) {
   extern __shared__ unsigned char smem_raw[];
   T *const line = reinterpret_cast<T *>(smem_raw);
-  for (int64_t t = blockIdx.x; t < lines; t += gridDim.x) {
+  for (int64_t t = fused::xcd_tile(blockIdx.x, gridDim.x); t < lines; t += gridDim.x) {
     TileRef const ref = grids(t);
     int64_t const a = ref.local / ref.B;
     int64_t const b = ref.local - a * ref.B;
