@@ -118,19 +118,28 @@ inline int fused_chunk(int dtype, int L) {
   return fused_chunk_of(dtype == PALIWA_FLOAT64 ? 8 : 4, L);
 }
 // Workgroup size of the fused kernel; wider workgroups along strided axes
-// give each tile row a longer run of consecutive points.
+// give each tile row a longer run of consecutive points.  Float64 lines of
+// 512 points (G = 32) take 1024 threads: 32-point (256-byte) tile rows in
+// 131 KB of LDS, one workgroup per CU; with 512 threads the rows were 128
+// bytes and the sweep of a 513^3 grid took 572 instead of 445 us (4-byte types
+// keep 512: their int32 ladder would spill at 1024).
 #ifndef PALIWA_STRIDED_THREADS
 #define PALIWA_STRIDED_THREADS 512
 #endif
 #ifndef PALIWA_STRIDED_THREADS_LONG
 #define PALIWA_STRIDED_THREADS_LONG PALIWA_STRIDED_THREADS
 #endif
+#ifndef PALIWA_STRIDED_THREADS_G32_F64
+#define PALIWA_STRIDED_THREADS_G32_F64 1024
+#endif
 #ifndef PALIWA_CONTIG_THREADS
 #define PALIWA_CONTIG_THREADS 256
 #endif
-constexpr int fused_threads(bool strided, int G) {
+constexpr int fused_threads(bool strided, int G, int elem_bytes) {
   if (!strided)
     return G >= 4 ? PALIWA_CONTIG_THREADS : 256;
+  if (G == 32 && elem_bytes == 8)
+    return PALIWA_STRIDED_THREADS_G32_F64;
   if (G >= 32)
     return PALIWA_STRIDED_THREADS_LONG;
   return G >= 4 ? PALIWA_STRIDED_THREADS : 256;
@@ -138,7 +147,7 @@ constexpr int fused_threads(bool strided, int G) {
 // Lines per tile of the fused kernel.
 inline int fused_tile_lines(int dtype, int L, bool strided) {
   int const G = (1 << L) / fused_chunk(dtype, L);
-  return fused_threads(strided, G) / G;
+  return fused_threads(strided, G, dtype == PALIWA_FLOAT64 ? 8 : 4) / G;
 }
 
 // Batched fused sweep: `tiles` tiles of the grids in `grids`, all with line
