@@ -362,9 +362,11 @@ def per_axis_benchmark(torch, grid, level, wavelet, reps=20):
 
 
 def other_workloads(torch, fields):
-    """The 2-D hat grid (1025^2) and the anisotropic 6-D grid (5,5,4,4,3,3)
-    with both wavelets: per-axis GB/s.  Their results are bit-identical to
-    the CPU oracle (tests/test_workloads_gpu.py)."""
+    """The 2-D hat grid (1025^2), the anisotropic 6-D grid (5,5,4,4,3,3)
+    with both wavelets and the 513^3 float64 biorthogonal grid (larger than
+    the Infinity Cache): per-axis GB/s.  The first two are bit-identical to
+    the CPU oracle (tests/test_workloads_gpu.py); the 513^3 grid runs the same
+    kernels at period 512 (tests/test_parity_gpu.py covers every period)."""
     out = {}
     g = fields.gaussian_bump_2d((10, 10), seed=0, device="cuda")
     out["hat_2d_1025"] = per_axis_benchmark(torch, g, [10, 10], "hat")
@@ -372,6 +374,11 @@ def other_workloads(torch, fields):
     for wavelet in ("hat", "biorthogonal"):
         out[f"{wavelet}_6d_33x33x17x17x9x9"] = per_axis_benchmark(
             torch, g, [5, 5, 4, 4, 3, 3], wavelet)
+    del g
+    # a grid four times the Infinity Cache (1.08 GB): every sweep streams HBM
+    g = fields.fourier_modes((9, 9, 9), modes=16, kmax=32, seed=1, device="cuda")
+    out["biorthogonal_3d_513"] = per_axis_benchmark(torch, g, [9, 9, 9], "biorthogonal")
+    del g
     return out
 
 
