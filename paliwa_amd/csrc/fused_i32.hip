@@ -11,7 +11,7 @@ using namespace fused;
 
 template <int K, int G, bool STRIDED>
 hipError_t launch_int_layout(SweepParams const &p, hipStream_t stream) {
-  using Geom = TileGeom<K, G, STRIDED, 4>;
+  using Geom = TileGeom<K, G, STRIDED, int32_t>;
   int64_t const lines = p.A * p.B;
   int64_t const tiles = (lines + Geom::TB - 1) / Geom::TB;
   size_t const bytes = sizeof(int32_t) * Geom::lds_elems(p.m, STRIDED);

@@ -53,7 +53,9 @@ constexpr bool kStridedDirect = false;
 // columns and every row of the tile is one run of NT/G consecutive points in
 // memory; wider runs waste less of the cache lines at their ends (rows start
 // at any 8-byte offset when the inner extent is odd, as with periodic images).
-constexpr int threads_for(bool strided, int G, int ES) { return fused_threads(strided, G, ES); }
+template <typename T> constexpr int threads_for(bool strided, int G) {
+  return fused_threads(strided, G, int(sizeof(T)), std::is_integral_v<T>);
+}
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 
@@ -229,7 +231,7 @@ __device__ __forceinline__ void run_generic(T (&v)[K], int gbase, int gi,
   }
 }
 
-// LDS geometry of a tile of TB lines of ES-byte points.
+// LDS geometry of a tile of TB lines of ES-byte points (ES = sizeof(T)).
 //   contiguous axis: line-major, point r of line l at l*LP + r + (r >> LOGK)*PAD
 //   strided axis:    row-major,  point r of column c at r*TB + c + (r >> LOGK)*PAD
 // PAD points after every chunk of K rows put the register loads of a
@@ -240,8 +242,9 @@ __device__ __forceinline__ void run_generic(T (&v)[K], int gbase, int gi,
 // all banks; 1 from G = 16 (float64) or 32 (float32) up.  With one lane per
 // line (G = 1) the lanes are a line pitch apart, which is odd: no pad, so that
 // the staging copy of lines with an image (odd m) runs without gaps.
-template <int K, int G, bool STRIDED, int ES> struct TileGeom {
-  static constexpr int NT = threads_for(STRIDED, G, ES);
+template <int K, int G, bool STRIDED, typename T> struct TileGeom {
+  static constexpr int ES = int(sizeof(T));
+  static constexpr int NT = threads_for<T>(STRIDED, G);
   static constexpr int TB = NT / G;
   static constexpr int LOGK = ilog2(K);
   static constexpr int PAD = G == 1 ? 0 : (128 / (ES * G) < 1 ? 1 : 128 / (ES * G));
@@ -277,7 +280,7 @@ template <typename T, int K, int G, bool STRIDED>
 __device__ __forceinline__ Stage<T> load_tile(SweepParams const &p,
                                               TileRef const &ref,
                                               T (&buf)[K + 1]) {
-  using Geom = TileGeom<K, G, STRIDED, sizeof(T)>;
+  using Geom = TileGeom<K, G, STRIDED, T>;
   constexpr int TB = Geom::TB;
   int const tid = threadIdx.x;
   int const m = p.m;
@@ -324,9 +327,9 @@ __device__ __forceinline__ Stage<T> load_tile(SweepParams const &p,
   return st;
 }
 
-template <int K, int G, bool STRIDED, int ES>
+template <int K, int G, bool STRIDED, typename T>
 __device__ __forceinline__ int lds_index(int line, int r, int LP) {
-  using Geom = TileGeom<K, G, STRIDED, ES>;
+  using Geom = TileGeom<K, G, STRIDED, T>;
   if constexpr (STRIDED)
     return r * Geom::TB + line + (r >> Geom::LOGK) * Geom::PAD;
   else
@@ -339,7 +342,7 @@ __device__ __forceinline__ void stash_tile(SweepParams const &p,
                                            Stage<T> const &st,
                                            T const (&buf)[K + 1], T *lds,
                                            uint32_t magic_m, int LP) {
-  using Geom = TileGeom<K, G, STRIDED, sizeof(T)>;
+  using Geom = TileGeom<K, G, STRIDED, T>;
   int const tid = threadIdx.x;
   int const m = p.m;
   if constexpr (!STRIDED) {
@@ -349,7 +352,7 @@ __device__ __forceinline__ void stash_tile(SweepParams const &p,
       if ((st.full && i < K) || e < st.count) {
         int const l = __umulhi(static_cast<uint32_t>(e), magic_m);
         int const r = e - l * m;
-        lds[lds_index<K, G, false, sizeof(T)>(l, r, LP)] = buf[i];
+        lds[lds_index<K, G, false, T>(l, r, LP)] = buf[i];
       }
     }
   } else {
@@ -358,9 +361,9 @@ __device__ __forceinline__ void stash_tile(SweepParams const &p,
     if (st.lane != nullptr) {
 #pragma unroll
       for (int i = 0; i < K; ++i)
-        lds[lds_index<K, G, true, sizeof(T)>(col, row0 + i * G, LP)] = buf[i];
+        lds[lds_index<K, G, true, T>(col, row0 + i * G, LP)] = buf[i];
       if (st.image != 0)
-        lds[lds_index<K, G, true, sizeof(T)>(col, 1 << p.L, LP)] = buf[K];
+        lds[lds_index<K, G, true, T>(col, 1 << p.L, LP)] = buf[K];
     }
   }
 }
@@ -370,7 +373,7 @@ template <typename T, int K, int G, bool STRIDED>
 __device__ __forceinline__ void store_tile(SweepParams const &p,
                                            Stage<T> const &st, T const *lds,
                                            uint32_t magic_m, int LP) {
-  using Geom = TileGeom<K, G, STRIDED, sizeof(T)>;
+  using Geom = TileGeom<K, G, STRIDED, T>;
   int const tid = threadIdx.x;
   int const m = p.m;
   T out[K + 1];
@@ -381,7 +384,7 @@ __device__ __forceinline__ void store_tile(SweepParams const &p,
       if ((st.full && i < K) || e < st.count) {
         int const l = __umulhi(static_cast<uint32_t>(e), magic_m);
         int const r = e - l * m;
-        out[i] = lds[lds_index<K, G, false, sizeof(T)>(l, r, LP)];
+        out[i] = lds[lds_index<K, G, false, T>(l, r, LP)];
       }
     }
     T *const s = st.src + tid;
@@ -403,9 +406,9 @@ __device__ __forceinline__ void store_tile(SweepParams const &p,
     if (st.lane != nullptr) {
 #pragma unroll
       for (int i = 0; i < K; ++i)
-        out[i] = lds[lds_index<K, G, true, sizeof(T)>(col, row0 + i * G, LP)];
+        out[i] = lds[lds_index<K, G, true, T>(col, row0 + i * G, LP)];
       if (st.image != 0)
-        out[K] = lds[lds_index<K, G, true, sizeof(T)>(col, 1 << p.L, LP)];
+        out[K] = lds[lds_index<K, G, true, T>(col, 1 << p.L, LP)];
 #pragma unroll
       for (int i = 0; i < K; ++i)
         st.lane[i * st.step] = out[i];
@@ -425,7 +428,7 @@ template <typename T, int K, int G, bool STRIDED, int MODE, int PAT, bool ASC,
 __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
                                             Src const &src, uint32_t magic_m,
                                             int64_t ntiles, T *lds) {
-  using Geom = TileGeom<K, G, STRIDED, sizeof(T)>;
+  using Geom = TileGeom<K, G, STRIDED, T>;
   constexpr int TB = Geom::TB;
   int const tid = threadIdx.x;
   int const m = p.m;
@@ -522,7 +525,7 @@ __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
       T v[K];
 #pragma unroll
       for (int j = 0; j < K; ++j)
-        v[j] = lds[lds_index<K, G, STRIDED, sizeof(T)>(li, gi * K + j, LP)];
+        v[j] = lds[lds_index<K, G, STRIDED, T>(li, gi * K + j, LP)];
       if constexpr (kInt) {
         // lines past the grid's end (last tile) hold stale LDS words; they
         // are never stored, but their arithmetic must not raise the overflow
@@ -538,9 +541,9 @@ __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
       compute(v);
 #pragma unroll
       for (int j = 0; j < K; ++j)
-        lds[lds_index<K, G, STRIDED, sizeof(T)>(li, gi * K + j, LP)] = v[j];
+        lds[lds_index<K, G, STRIDED, T>(li, gi * K + j, LP)] = v[j];
       if (m != n && gi == 0)
-        lds[lds_index<K, G, STRIDED, sizeof(T)>(li, n, LP)] = v[0];
+        lds[lds_index<K, G, STRIDED, T>(li, n, LP)] = v[0];
       __syncthreads();
       store_tile<T, K, G, STRIDED>(p, cur, lds, magic_m, LP);
       cur = nxt;
@@ -554,7 +557,7 @@ __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
 }
 
 template <typename T, int K, int G, bool STRIDED, int PAT, bool ASC>
-__global__ __launch_bounds__(threads_for(STRIDED, G, sizeof(T))) void fused_ladder(SweepParams p,
+__global__ __launch_bounds__(threads_for<T>(STRIDED, G)) void fused_ladder(SweepParams p,
                                                          uint32_t magic_m,
                                                          int64_t ntiles) {
   extern __shared__ unsigned char smem_raw[];
@@ -566,7 +569,7 @@ __global__ __launch_bounds__(threads_for(STRIDED, G, sizeof(T))) void fused_ladd
 // The same sweep over the tiles of many grids with equal line length along
 // the axis (one launch of a batched transform).
 template <typename T, int K, int G, bool STRIDED, int PAT, bool ASC>
-__global__ __launch_bounds__(threads_for(STRIDED, G, sizeof(T))) void fused_ladder_batched(
+__global__ __launch_bounds__(threads_for<T>(STRIDED, G)) void fused_ladder_batched(
     SweepParams p, uint32_t magic_m, int64_t ntiles, ManyGrids grids) {
   extern __shared__ unsigned char smem_raw[];
   sweep_tiles<T, K, G, STRIDED, 0, PAT, ASC>(p, grids, magic_m, ntiles,
@@ -574,7 +577,7 @@ __global__ __launch_bounds__(threads_for(STRIDED, G, sizeof(T))) void fused_ladd
 }
 
 template <typename T, int K, int G, bool STRIDED>
-__global__ __launch_bounds__(threads_for(STRIDED, G, sizeof(T))) void fused_generic(SweepParams p,
+__global__ __launch_bounds__(threads_for<T>(STRIDED, G)) void fused_generic(SweepParams p,
                                                           uint32_t magic_m,
                                                           int64_t ntiles) {
   extern __shared__ unsigned char smem_raw[];
@@ -611,7 +614,7 @@ hipError_t launch_one(Kernel kernel, SweepParams const &p, size_t bytes,
 
 template <typename T, int K, int G, bool STRIDED>
 hipError_t launch_kg_layout(SweepParams const &p, hipStream_t stream) {
-  using Geom = TileGeom<K, G, STRIDED, sizeof(T)>;
+  using Geom = TileGeom<K, G, STRIDED, T>;
   int64_t const lines = p.A * p.B;
   int64_t const tiles = (lines + Geom::TB - 1) / Geom::TB;
   size_t const bytes = sizeof(T) * Geom::lds_elems(p.m, STRIDED);
@@ -645,7 +648,7 @@ hipError_t launch_kg(SweepParams const &p, hipStream_t stream) {
 template <typename T, int K, int G, bool STRIDED>
 hipError_t launch_batched_layout(SweepParams const &p, ManyGrids grids,
                                  int64_t tiles, hipStream_t stream) {
-  using Geom = TileGeom<K, G, STRIDED, sizeof(T)>;
+  using Geom = TileGeom<K, G, STRIDED, T>;
   size_t const bytes = sizeof(T) * Geom::lds_elems(p.m, STRIDED);
   bool const asc = p.schedule == kScheduleLadderAsc;
   if (p.schedule != kScheduleLadderAsc && p.schedule != kScheduleLadderDesc)

@@ -202,15 +202,15 @@ struct FusedPartialArgs {
 // here for every G, G = 1 included), then the row table (int16 per row).
 template <typename T, int K, int G>
 constexpr size_t fused_partial_table_at() {
-  using Geom = fused::TileGeom<K, G, true, sizeof(T)>;
+  using Geom = fused::TileGeom<K, G, true, T>;
   return (sizeof(T) * size_t(K * G * Geom::TB + G * Geom::PAD + 2) + 15) / 16 * 16;
 }
 
 template <typename T, int K, int G, int PAT, bool ASC>
-__global__ __launch_bounds__(fused::threads_for(true, G, sizeof(T))) void partial_fused(
+__global__ __launch_bounds__(fused::threads_for<T>(true, G)) void partial_fused(
     FusedPartialArgs a, T *__restrict__ local, T const *__restrict__ ghosts,
     int64_t ntiles) {
-  using Geom = fused::TileGeom<K, G, true, sizeof(T)>;
+  using Geom = fused::TileGeom<K, G, true, T>;
   constexpr int TB = Geom::TB;
   constexpr int NT = Geom::NT;
   constexpr int P = K * G;
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(fused::threads_for(true, G, sizeof(T))) void partia
     __syncthreads(); // the previous tile's output reads are done
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      lds[fused::lds_index<K, G, true, sizeof(T)>(c, r0 + k * RS, 0)] = buf[k];
+      lds[fused::lds_index<K, G, true, T>(c, r0 + k * RS, 0)] = buf[k];
     __syncthreads();
     // the next tile's loads stay in flight while this one computes
     Column const nxt = column(t + nwg);
@@ -282,11 +282,11 @@ __global__ __launch_bounds__(fused::threads_for(true, G, sizeof(T))) void partia
     T v[K];
 #pragma unroll
     for (int j = 0; j < K; ++j)
-      v[j] = lds[fused::lds_index<K, G, true, sizeof(T)>(li, gi * K + j, 0)];
+      v[j] = lds[fused::lds_index<K, G, true, T>(li, gi * K + j, 0)];
     fused::run_ladder<T, K, G, PAT, ASC>(v, lane_base, gi, a.lo, a.hi, cf);
 #pragma unroll
     for (int j = 0; j < K; ++j)
-      lds[fused::lds_index<K, G, true, sizeof(T)>(li, gi * K + j, 0)] = v[j];
+      lds[fused::lds_index<K, G, true, T>(li, gi * K + j, 0)] = v[j];
     __syncthreads();
     if (cur.l != nullptr) {
       for (int i = r0; i < a.out_count; i += RS) {
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(fused::threads_for(true, G, sizeof(T))) void partia
         int64_t j = a.out_local + i; // wraps only when the local run is the whole period
         if (j >= a.m_local)
           j -= a.m_local;
-        cur.l[j * a.B] = lds[fused::lds_index<K, G, true, sizeof(T)>(c, x, 0)];
+        cur.l[j * a.B] = lds[fused::lds_index<K, G, true, T>(c, x, 0)];
       }
     }
     cur = nxt;
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(fused::threads_for(true, G, sizeof(T))) void partia
 template <typename T, int K, int G, int PAT, bool ASC>
 hipError_t launch_partial_fused_kg(FusedPartialArgs const &a, T *local, T const *ghosts,
                                    hipStream_t s) {
-  using Geom = fused::TileGeom<K, G, true, sizeof(T)>;
+  using Geom = fused::TileGeom<K, G, true, T>;
   size_t const bytes = fused_partial_table_at<T, K, G>() + sizeof(int16_t) * K * G;
   auto kernel = partial_fused<T, K, G, PAT, ASC>;
   if (bytes > 65536) {
