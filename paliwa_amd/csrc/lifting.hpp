@@ -119,11 +119,12 @@ inline int fused_chunk(int dtype, int L) {
 }
 // Workgroup size of the fused kernel; wider workgroups along strided axes
 // give each tile row a longer run of consecutive points.  Float64 lines of
-// 512 points (G = 32) and float32 lines of 512 (G = 16) take 1024 threads:
-// 256-byte tile rows in 131 KB of LDS, one workgroup per CU; with 512 threads
-// the rows were 128 bytes and the strided sweeps of a 513^3 grid took 572
-// instead of 440 us (float64) and 249 instead of 230 us (float32).  The int32
-// ladder keeps 512 (it would spill at 1024).
+// 512 points (G = 32) and float32 lines of 512 and 1024 (G = 16, 32) take
+// 1024 threads: tile rows of 256 (128) bytes in 131 KB of LDS, one workgroup
+// per CU; with 512 threads the rows were half as long and the strided sweeps
+// of a 513^3 grid took 572 instead of 440 us (float64) and 249 instead of 230
+// us (float32), those of 1025-point float32 lines 307 instead of 274 us.  The
+// int32 ladder keeps 512 (it would spill at 1024).
 #ifndef PALIWA_STRIDED_THREADS
 #define PALIWA_STRIDED_THREADS 512
 #endif
@@ -136,6 +137,9 @@ inline int fused_chunk(int dtype, int L) {
 #ifndef PALIWA_STRIDED_THREADS_G16_F32
 #define PALIWA_STRIDED_THREADS_G16_F32 1024
 #endif
+#ifndef PALIWA_STRIDED_THREADS_G32_F32
+#define PALIWA_STRIDED_THREADS_G32_F32 1024
+#endif
 #ifndef PALIWA_CONTIG_THREADS
 #define PALIWA_CONTIG_THREADS 256
 #endif
@@ -146,6 +150,8 @@ constexpr int fused_threads(bool strided, int G, int elem_bytes, bool integral =
     return PALIWA_STRIDED_THREADS_G32_F64;
   if (G == 16 && elem_bytes == 4 && !integral)
     return PALIWA_STRIDED_THREADS_G16_F32;
+  if (G == 32 && elem_bytes == 4 && !integral)
+    return PALIWA_STRIDED_THREADS_G32_F32;
   if (G >= 32)
     return PALIWA_STRIDED_THREADS_LONG;
   return G >= 4 ? PALIWA_STRIDED_THREADS : 256;

@@ -57,7 +57,7 @@ def _run(pp, full, level, dim, levels, filters, local_start, count, out_off, out
 
 @pytest.mark.parametrize("shape_levels,dim", [((9,), 0), ((7, 5), 0), ((7, 5), 1),
                                               ((4, 6, 3), 1), ((4, 6, 3), 2), ((3, 5), 0),
-                                              ((9, 4), 0), ((11, 2), 0)])
+                                              ((9, 4), 0), ((10, 3), 0), ((11, 2), 0)])
 def test_blocks_with_ghosts(pp, shape_levels, dim):
     level = list(shape_levels)
     L = level[dim]
