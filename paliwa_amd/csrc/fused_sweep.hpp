@@ -53,8 +53,8 @@ constexpr bool kStridedDirect = false;
 // columns and every row of the tile is one run of NT/G consecutive points in
 // memory; wider runs waste less of the cache lines at their ends (rows start
 // at any 8-byte offset when the inner extent is odd, as with periodic images).
-template <typename T> constexpr int threads_for(bool strided, int G) {
-  return fused_threads(strided, G, int(sizeof(T)), std::is_integral_v<T>);
+template <typename T> constexpr int threads_for(bool strided, int K, int G) {
+  return fused_threads(strided, K, G, int(sizeof(T)), std::is_integral_v<T>);
 }
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
@@ -244,7 +244,7 @@ __device__ __forceinline__ void run_generic(T (&v)[K], int gbase, int gi,
 // the staging copy of lines with an image (odd m) runs without gaps.
 template <int K, int G, bool STRIDED, typename T> struct TileGeom {
   static constexpr int ES = int(sizeof(T));
-  static constexpr int NT = threads_for<T>(STRIDED, G);
+  static constexpr int NT = threads_for<T>(STRIDED, K, G);
   static constexpr int TB = NT / G;
   static constexpr int LOGK = ilog2(K);
   static constexpr int PAD = G == 1 ? 0 : (128 / (ES * G) < 1 ? 1 : 128 / (ES * G));
@@ -557,7 +557,7 @@ __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
 }
 
 template <typename T, int K, int G, bool STRIDED, int PAT, bool ASC>
-__global__ __launch_bounds__(threads_for<T>(STRIDED, G)) void fused_ladder(SweepParams p,
+__global__ __launch_bounds__(threads_for<T>(STRIDED, K, G)) void fused_ladder(SweepParams p,
                                                          uint32_t magic_m,
                                                          int64_t ntiles) {
   extern __shared__ unsigned char smem_raw[];
@@ -569,7 +569,7 @@ __global__ __launch_bounds__(threads_for<T>(STRIDED, G)) void fused_ladder(Sweep
 // The same sweep over the tiles of many grids with equal line length along
 // the axis (one launch of a batched transform).
 template <typename T, int K, int G, bool STRIDED, int PAT, bool ASC>
-__global__ __launch_bounds__(threads_for<T>(STRIDED, G)) void fused_ladder_batched(
+__global__ __launch_bounds__(threads_for<T>(STRIDED, K, G)) void fused_ladder_batched(
     SweepParams p, uint32_t magic_m, int64_t ntiles, ManyGrids grids) {
   extern __shared__ unsigned char smem_raw[];
   sweep_tiles<T, K, G, STRIDED, 0, PAT, ASC>(p, grids, magic_m, ntiles,
@@ -577,7 +577,7 @@ __global__ __launch_bounds__(threads_for<T>(STRIDED, G)) void fused_ladder_batch
 }
 
 template <typename T, int K, int G, bool STRIDED>
-__global__ __launch_bounds__(threads_for<T>(STRIDED, G)) void fused_generic(SweepParams p,
+__global__ __launch_bounds__(threads_for<T>(STRIDED, K, G)) void fused_generic(SweepParams p,
                                                           uint32_t magic_m,
                                                           int64_t ntiles) {
   extern __shared__ unsigned char smem_raw[];

@@ -118,48 +118,43 @@ inline int fused_chunk(int dtype, int L) {
   return fused_chunk_of(dtype == PALIWA_FLOAT64 ? 8 : 4, L);
 }
 // Workgroup size of the fused kernel; wider workgroups along strided axes
-// give each tile row a longer run of consecutive points.  Float64 lines of
-// 512 points (G = 32) and float32 lines of 512 and 1024 (G = 16, 32) take
-// 1024 threads: tile rows of 256 (128) bytes in 131 KB of LDS, one workgroup
-// per CU; with 512 threads the rows were half as long and the strided sweeps
-// of a 513^3 grid took 572 instead of 440 us (float64) and 249 instead of 230
-// us (float32), those of 1025-point float32 lines 307 instead of 274 us.  The
-// int32 ladder keeps 512 (it would spill at 1024).
+// give each tile row a longer run of consecutive points.  A strided float
+// sweep whose 512-thread tile rows would be shorter than 256 bytes takes 1024
+// threads when the doubled tile still fits in LDS (one workgroup per CU):
+// float64 lines of 512 and 1024 points, float32 lines of 512 to 2048.  With
+// 512 threads the strided sweeps of a 513^3 grid took 572 instead of 440 us
+// (float64) and 249 instead of 230 us (float32), those of 1025-point float32
+// lines 307 instead of 274 us.  Longer rows do not pay where they already
+// are 256 bytes (257^3 float64: 53 us against 50.5 us), and the int32 ladder
+// keeps 512 (it would spill at 1024).
 #ifndef PALIWA_STRIDED_THREADS
 #define PALIWA_STRIDED_THREADS 512
 #endif
 #ifndef PALIWA_STRIDED_THREADS_LONG
 #define PALIWA_STRIDED_THREADS_LONG PALIWA_STRIDED_THREADS
 #endif
-#ifndef PALIWA_STRIDED_THREADS_G32_F64
-#define PALIWA_STRIDED_THREADS_G32_F64 1024
-#endif
-#ifndef PALIWA_STRIDED_THREADS_G16_F32
-#define PALIWA_STRIDED_THREADS_G16_F32 1024
-#endif
-#ifndef PALIWA_STRIDED_THREADS_G32_F32
-#define PALIWA_STRIDED_THREADS_G32_F32 1024
+#ifndef PALIWA_WIDE_TILE_LDS
+#define PALIWA_WIDE_TILE_LDS (136 * 1024)
 #endif
 #ifndef PALIWA_CONTIG_THREADS
 #define PALIWA_CONTIG_THREADS 256
 #endif
-constexpr int fused_threads(bool strided, int G, int elem_bytes, bool integral = false) {
+constexpr int fused_threads(bool strided, int K, int G, int elem_bytes,
+                            bool integral = false) {
   if (!strided)
     return G >= 4 ? PALIWA_CONTIG_THREADS : 256;
-  if (G == 32 && elem_bytes == 8)
-    return PALIWA_STRIDED_THREADS_G32_F64;
-  if (G == 16 && elem_bytes == 4 && !integral)
-    return PALIWA_STRIDED_THREADS_G16_F32;
-  if (G == 32 && elem_bytes == 4 && !integral)
-    return PALIWA_STRIDED_THREADS_G32_F32;
-  if (G >= 32)
-    return PALIWA_STRIDED_THREADS_LONG;
-  return G >= 4 ? PALIWA_STRIDED_THREADS : 256;
+  int const base = G >= 32 ? PALIWA_STRIDED_THREADS_LONG : (G >= 4 ? PALIWA_STRIDED_THREADS : 256);
+  if (integral || base != 512)
+    return base;
+  long const rows = 512 / G;                                   // tile width at 512
+  long const wide = 2 * (long(K) * G + 1) * rows * elem_bytes; // LDS at 1024
+  return rows * elem_bytes < 256 && wide <= PALIWA_WIDE_TILE_LDS ? 1024 : base;
 }
 // Lines per tile of the fused kernel.
 inline int fused_tile_lines(int dtype, int L, bool strided) {
-  int const G = (1 << L) / fused_chunk(dtype, L);
-  return fused_threads(strided, G, dtype == PALIWA_FLOAT64 ? 8 : 4) / G;
+  int const K = fused_chunk(dtype, L);
+  int const G = (1 << L) / K;
+  return fused_threads(strided, K, G, dtype == PALIWA_FLOAT64 ? 8 : 4) / G;
 }
 
 // Batched fused sweep: `tiles` tiles of the grids in `grids`, all with line

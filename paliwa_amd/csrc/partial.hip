@@ -207,7 +207,7 @@ constexpr size_t fused_partial_table_at() {
 }
 
 template <typename T, int K, int G, int PAT, bool ASC>
-__global__ __launch_bounds__(fused::threads_for<T>(true, G)) void partial_fused(
+__global__ __launch_bounds__(fused::threads_for<T>(true, K, G)) void partial_fused(
     FusedPartialArgs a, T *__restrict__ local, T const *__restrict__ ghosts,
     int64_t ntiles) {
   using Geom = fused::TileGeom<K, G, true, T>;
