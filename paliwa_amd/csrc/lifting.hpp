@@ -121,12 +121,15 @@ inline int fused_chunk(int dtype, int L) {
 // give each tile row a longer run of consecutive points.  A strided float
 // sweep whose 512-thread tile rows would be shorter than 256 bytes takes 1024
 // threads when the doubled tile still fits in LDS (one workgroup per CU):
-// float64 lines of 512 and 1024 points, float32 lines of 512 to 2048.  With
+// float64 lines of 512 points, float32 lines of 512 and 1024.  With
 // 512 threads the strided sweeps of a 513^3 grid took 572 instead of 440 us
 // (float64) and 249 instead of 230 us (float32), those of 1025-point float32
 // lines 307 instead of 274 us.  Longer rows do not pay where they already
 // are 256 bytes (257^3 float64: 53 us against 50.5 us), and the int32 ladder
-// keeps 512 (it would spill at 1024).
+// keeps 512 (it would spill at 1024).  Lines of 64 chunks (G = 64) keep 512:
+// a wide tile halves the tile count, and the 1025^2 float64 grid's strided
+// sweep (65 wide tiles for 256 CUs) took 11.5 instead of 8.2 us, although
+// large grids with such lines gain 13-15% from it.
 #ifndef PALIWA_STRIDED_THREADS
 #define PALIWA_STRIDED_THREADS 512
 #endif
@@ -144,7 +147,7 @@ constexpr int fused_threads(bool strided, int K, int G, int elem_bytes,
   if (!strided)
     return G >= 4 ? PALIWA_CONTIG_THREADS : 256;
   int const base = G >= 32 ? PALIWA_STRIDED_THREADS_LONG : (G >= 4 ? PALIWA_STRIDED_THREADS : 256);
-  if (integral || base != 512)
+  if (integral || base != 512 || G > 32)
     return base;
   long const rows = 512 / G;                                   // tile width at 512
   long const wide = 2 * (long(K) * G + 1) * rows * elem_bytes; // LDS at 1024
