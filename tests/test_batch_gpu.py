@@ -90,6 +90,33 @@ def test_batch_six_dimensions(pw):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batch_wide_tiles(pw, dtype):
+    """Strided axes with lines of 512 and 1024 points, whose tiles take 1024
+    threads (float64 at 512, float32 at 512 and 1024): every grid against the
+    oracle, both directions, hat and biorthogonal."""
+    levels = [(9, 3), (10, 2), (9, 9), (3, 10), (10, 4)]
+    lmin, lmax = [0, 0], [10, 10]
+    arrays = make_grids(levels, True, dtype, seed=24)
+    for name in ("hat", "biorthogonal"):
+        grids = [torch.from_numpy(a).cuda() for a in arrays]
+        batch = pw.CombinationBatch(grids, levels, lmin, lmax)
+        batch.hierarchize(name)
+        torch.cuda.synchronize()
+        wants = []
+        for a, g, l in zip(arrays, grids, levels):
+            want = a.copy()
+            ob.hierarchize(want, list(l), lmin, lmax, name)
+            wants.append(want)
+            assert np.array_equal(g.cpu().numpy(), want), (name, l)
+        batch.dehierarchize(name)
+        torch.cuda.synchronize()
+        for want, g, l in zip(wants, grids, levels):
+            ob.dehierarchize(want, list(l), lmin, lmax, name)
+            assert np.array_equal(g.cpu().numpy(), want), (name, l)
+        batch.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_batch_long_lines(pw, dtype):
     """Lines longer than the fused kernel holds (2^12 and 2^13 float64, 2^13
     float32) go through one batched LDS-line launch per group: every grid
