@@ -242,9 +242,9 @@ __device__ __forceinline__ void run_generic(T (&v)[K], int gbase, int gi,
 // all banks; 1 from G = 16 (float64) or 32 (float32) up.  With one lane per
 // line (G = 1) the lanes are a line pitch apart, which is odd: no pad, so that
 // the staging copy of lines with an image (odd m) runs without gaps.
-template <int K, int G, bool STRIDED, typename T> struct TileGeom {
+template <int K, int G, bool STRIDED, typename T, int NTO = 0> struct TileGeom {
   static constexpr int ES = int(sizeof(T));
-  static constexpr int NT = threads_for<T>(STRIDED, K, G);
+  static constexpr int NT = NTO ? NTO : threads_for<T>(STRIDED, K, G); // NTO: override
   static constexpr int TB = NT / G;
   static constexpr int LOGK = ilog2(K);
   static constexpr int PAD = G == 1 ? 0 : (128 / (ES * G) < 1 ? 1 : 128 / (ES * G));
@@ -276,11 +276,11 @@ template <typename T> struct Stage {
 
 // Issue every HBM load of one tile for this thread (K+1 per lane at most,
 // all in flight together).
-template <typename T, int K, int G, bool STRIDED>
+template <typename T, int K, int G, bool STRIDED, int NTO = 0>
 __device__ __forceinline__ Stage<T> load_tile(SweepParams const &p,
                                               TileRef const &ref,
                                               T (&buf)[K + 1]) {
-  using Geom = TileGeom<K, G, STRIDED, T>;
+  using Geom = TileGeom<K, G, STRIDED, T, NTO>;
   constexpr int TB = Geom::TB;
   int const tid = threadIdx.x;
   int const m = p.m;
@@ -327,9 +327,9 @@ __device__ __forceinline__ Stage<T> load_tile(SweepParams const &p,
   return st;
 }
 
-template <int K, int G, bool STRIDED, typename T>
+template <int K, int G, bool STRIDED, typename T, int NTO = 0>
 __device__ __forceinline__ int lds_index(int line, int r, int LP) {
-  using Geom = TileGeom<K, G, STRIDED, T>;
+  using Geom = TileGeom<K, G, STRIDED, T, NTO>;
   if constexpr (STRIDED)
     return r * Geom::TB + line + (r >> Geom::LOGK) * Geom::PAD;
   else
@@ -337,12 +337,12 @@ __device__ __forceinline__ int lds_index(int line, int r, int LP) {
 }
 
 // Registers holding a loaded tile -> LDS.
-template <typename T, int K, int G, bool STRIDED>
+template <typename T, int K, int G, bool STRIDED, int NTO = 0>
 __device__ __forceinline__ void stash_tile(SweepParams const &p,
                                            Stage<T> const &st,
                                            T const (&buf)[K + 1], T *lds,
                                            uint32_t magic_m, int LP) {
-  using Geom = TileGeom<K, G, STRIDED, T>;
+  using Geom = TileGeom<K, G, STRIDED, T, NTO>;
   int const tid = threadIdx.x;
   int const m = p.m;
   if constexpr (!STRIDED) {
@@ -352,7 +352,7 @@ __device__ __forceinline__ void stash_tile(SweepParams const &p,
       if ((st.full && i < K) || e < st.count) {
         int const l = __umulhi(static_cast<uint32_t>(e), magic_m);
         int const r = e - l * m;
-        lds[lds_index<K, G, false, T>(l, r, LP)] = buf[i];
+        lds[lds_index<K, G, false, T, NTO>(l, r, LP)] = buf[i];
       }
     }
   } else {
@@ -361,19 +361,19 @@ __device__ __forceinline__ void stash_tile(SweepParams const &p,
     if (st.lane != nullptr) {
 #pragma unroll
       for (int i = 0; i < K; ++i)
-        lds[lds_index<K, G, true, T>(col, row0 + i * G, LP)] = buf[i];
+        lds[lds_index<K, G, true, T, NTO>(col, row0 + i * G, LP)] = buf[i];
       if (st.image != 0)
-        lds[lds_index<K, G, true, T>(col, 1 << p.L, LP)] = buf[K];
+        lds[lds_index<K, G, true, T, NTO>(col, 1 << p.L, LP)] = buf[K];
     }
   }
 }
 
 // LDS -> HBM for one tile.
-template <typename T, int K, int G, bool STRIDED>
+template <typename T, int K, int G, bool STRIDED, int NTO = 0>
 __device__ __forceinline__ void store_tile(SweepParams const &p,
                                            Stage<T> const &st, T const *lds,
                                            uint32_t magic_m, int LP) {
-  using Geom = TileGeom<K, G, STRIDED, T>;
+  using Geom = TileGeom<K, G, STRIDED, T, NTO>;
   int const tid = threadIdx.x;
   int const m = p.m;
   T out[K + 1];
@@ -384,7 +384,7 @@ __device__ __forceinline__ void store_tile(SweepParams const &p,
       if ((st.full && i < K) || e < st.count) {
         int const l = __umulhi(static_cast<uint32_t>(e), magic_m);
         int const r = e - l * m;
-        out[i] = lds[lds_index<K, G, false, T>(l, r, LP)];
+        out[i] = lds[lds_index<K, G, false, T, NTO>(l, r, LP)];
       }
     }
     T *const s = st.src + tid;
@@ -406,9 +406,9 @@ __device__ __forceinline__ void store_tile(SweepParams const &p,
     if (st.lane != nullptr) {
 #pragma unroll
       for (int i = 0; i < K; ++i)
-        out[i] = lds[lds_index<K, G, true, T>(col, row0 + i * G, LP)];
+        out[i] = lds[lds_index<K, G, true, T, NTO>(col, row0 + i * G, LP)];
       if (st.image != 0)
-        out[K] = lds[lds_index<K, G, true, T>(col, 1 << p.L, LP)];
+        out[K] = lds[lds_index<K, G, true, T, NTO>(col, 1 << p.L, LP)];
 #pragma unroll
       for (int i = 0; i < K; ++i)
         st.lane[i * st.step] = out[i];
@@ -424,11 +424,11 @@ __device__ __forceinline__ void store_tile(SweepParams const &p,
 // computed and stored, so every workgroup keeps loads in flight throughout.
 // MODE 0 = ladder (PAT, ASC used), 1 = generic level/filter lists.
 template <typename T, int K, int G, bool STRIDED, int MODE, int PAT, bool ASC,
-          typename Src>
+          int NTO, typename Src>
 __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
                                             Src const &src, uint32_t magic_m,
                                             int64_t ntiles, T *lds) {
-  using Geom = TileGeom<K, G, STRIDED, T>;
+  using Geom = TileGeom<K, G, STRIDED, T, NTO>;
   constexpr int TB = Geom::TB;
   int const tid = threadIdx.x;
   int const m = p.m;
@@ -514,18 +514,18 @@ __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
     T buf[K + 1];
     Stage<T> cur;
     if (t < ntiles)
-      cur = load_tile<T, K, G, STRIDED>(p, src(t), buf);
+      cur = load_tile<T, K, G, STRIDED, NTO>(p, src(t), buf);
     for (; t < ntiles; t += nwg) {
       __syncthreads(); // the previous tile's LDS reads are done
-      stash_tile<T, K, G, STRIDED>(p, cur, buf, lds, magic_m, LP);
+      stash_tile<T, K, G, STRIDED, NTO>(p, cur, buf, lds, magic_m, LP);
       __syncthreads();
       Stage<T> nxt;
       if (t + nwg < ntiles)
-        nxt = load_tile<T, K, G, STRIDED>(p, src(t + nwg), buf);
+        nxt = load_tile<T, K, G, STRIDED, NTO>(p, src(t + nwg), buf);
       T v[K];
 #pragma unroll
       for (int j = 0; j < K; ++j)
-        v[j] = lds[lds_index<K, G, STRIDED, T>(li, gi * K + j, LP)];
+        v[j] = lds[lds_index<K, G, STRIDED, T, NTO>(li, gi * K + j, LP)];
       if constexpr (kInt) {
         // lines past the grid's end (last tile) hold stale LDS words; they
         // are never stored, but their arithmetic must not raise the overflow
@@ -541,11 +541,11 @@ __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
       compute(v);
 #pragma unroll
       for (int j = 0; j < K; ++j)
-        lds[lds_index<K, G, STRIDED, T>(li, gi * K + j, LP)] = v[j];
+        lds[lds_index<K, G, STRIDED, T, NTO>(li, gi * K + j, LP)] = v[j];
       if (m != n && gi == 0)
-        lds[lds_index<K, G, STRIDED, T>(li, n, LP)] = v[0];
+        lds[lds_index<K, G, STRIDED, T, NTO>(li, n, LP)] = v[0];
       __syncthreads();
-      store_tile<T, K, G, STRIDED>(p, cur, lds, magic_m, LP);
+      store_tile<T, K, G, STRIDED, NTO>(p, cur, lds, magic_m, LP);
       cur = nxt;
     }
   }
@@ -556,12 +556,11 @@ __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
   }
 }
 
-template <typename T, int K, int G, bool STRIDED, int PAT, bool ASC>
-__global__ __launch_bounds__(threads_for<T>(STRIDED, K, G)) void fused_ladder(SweepParams p,
-                                                         uint32_t magic_m,
-                                                         int64_t ntiles) {
+template <typename T, int K, int G, bool STRIDED, int PAT, bool ASC, int NTO = 0>
+__global__ __launch_bounds__(NTO ? NTO : threads_for<T>(STRIDED, K, G)) void fused_ladder(
+    SweepParams p, uint32_t magic_m, int64_t ntiles) {
   extern __shared__ unsigned char smem_raw[];
-  sweep_tiles<T, K, G, STRIDED, 0, PAT, ASC>(p, OneGrid{p.data, p.A, p.B},
+  sweep_tiles<T, K, G, STRIDED, 0, PAT, ASC, NTO>(p, OneGrid{p.data, p.A, p.B},
                                              magic_m, ntiles,
                                              reinterpret_cast<T *>(smem_raw));
 }
@@ -572,7 +571,7 @@ template <typename T, int K, int G, bool STRIDED, int PAT, bool ASC>
 __global__ __launch_bounds__(threads_for<T>(STRIDED, K, G)) void fused_ladder_batched(
     SweepParams p, uint32_t magic_m, int64_t ntiles, ManyGrids grids) {
   extern __shared__ unsigned char smem_raw[];
-  sweep_tiles<T, K, G, STRIDED, 0, PAT, ASC>(p, grids, magic_m, ntiles,
+  sweep_tiles<T, K, G, STRIDED, 0, PAT, ASC, 0>(p, grids, magic_m, ntiles,
                                              reinterpret_cast<T *>(smem_raw));
 }
 
@@ -581,7 +580,7 @@ __global__ __launch_bounds__(threads_for<T>(STRIDED, K, G)) void fused_generic(S
                                                           uint32_t magic_m,
                                                           int64_t ntiles) {
   extern __shared__ unsigned char smem_raw[];
-  sweep_tiles<T, K, G, STRIDED, 1, 0, true>(p, OneGrid{p.data, p.A, p.B},
+  sweep_tiles<T, K, G, STRIDED, 1, 0, true, 0>(p, OneGrid{p.data, p.A, p.B},
                                             magic_m, ntiles,
                                             reinterpret_cast<T *>(smem_raw));
 }
@@ -612,10 +611,45 @@ hipError_t launch_one(Kernel kernel, SweepParams const &p, size_t bytes,
   return hipGetLastError();
 }
 
+// Lines of 64 chunks keep 512-thread tiles by default (a wide tile halves the
+// tile count, which small grids cannot spare); a strided float sweep over many
+// lines (at least kWideTileLines: two rounds of 1024-thread tiles on 256 CUs)
+// takes the 1024-thread ladder instead, where its doubled tile fits in LDS
+// (float64 lines of 1024 points, float32 lines of 2048): tile rows of 128
+// instead of 64 bytes.
+constexpr int64_t kWideTileLines = 8192;
+template <typename T, int K, int G, bool STRIDED> constexpr bool wide_ladder() {
+  return STRIDED && !kStridedDirect && G == 64 && !std::is_integral_v<T> &&
+         threads_for<T>(STRIDED, K, G) == 512 &&
+         2 * (int64_t(K) * G + 1) * (512 / G) * int64_t(sizeof(T)) <= PALIWA_WIDE_TILE_LDS;
+}
+
 template <typename T, int K, int G, bool STRIDED>
 hipError_t launch_kg_layout(SweepParams const &p, hipStream_t stream) {
   using Geom = TileGeom<K, G, STRIDED, T>;
   int64_t const lines = p.A * p.B;
+  if constexpr (wide_ladder<T, K, G, STRIDED>()) {
+    if ((p.schedule == kScheduleLadderAsc || p.schedule == kScheduleLadderDesc) &&
+        lines >= kWideTileLines) {
+      using Wide = TileGeom<K, G, STRIDED, T, 1024>;
+      int64_t const tiles = (lines + Wide::TB - 1) / Wide::TB;
+      size_t const bytes = sizeof(T) * Wide::lds_elems(p.m, STRIDED);
+      bool const asc = p.schedule == kScheduleLadderAsc;
+      switch (p.pattern) {
+      case kPatOdd:
+        return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatOdd, true, 1024>, p, bytes, tiles, 1024, stream)
+                   : launch_one(fused_ladder<T, K, G, STRIDED, kPatOdd, false, 1024>, p, bytes, tiles, 1024, stream);
+      case kPatOddEven:
+        return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatOddEven, true, 1024>, p, bytes, tiles, 1024, stream)
+                   : launch_one(fused_ladder<T, K, G, STRIDED, kPatOddEven, false, 1024>, p, bytes, tiles, 1024, stream);
+      case kPatEvenOdd:
+        return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatEvenOdd, true, 1024>, p, bytes, tiles, 1024, stream)
+                   : launch_one(fused_ladder<T, K, G, STRIDED, kPatEvenOdd, false, 1024>, p, bytes, tiles, 1024, stream);
+      default:
+        break;
+      }
+    }
+  }
   int64_t const tiles = (lines + Geom::TB - 1) / Geom::TB;
   size_t const bytes = sizeof(T) * Geom::lds_elems(p.m, STRIDED);
   if (p.schedule == kScheduleLadderAsc || p.schedule == kScheduleLadderDesc) {

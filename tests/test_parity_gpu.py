@@ -129,6 +129,25 @@ def test_maximum_dimension_count(pw):
             assert_same(back, oracle_named(fwd, level, lmin, name, False), f"26-d {name} inv")
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_wide_tiles_over_many_lines(pw, dtype):
+    """Strided sweeps over 9225 lines of 64 chunks (float64 period 1024,
+    float32 period 2048), which take the 1024-thread ladder: both directions
+    against the oracle."""
+    rng = np.random.default_rng(17)
+    if dtype == np.float64:
+        level, extents = [10, 3, 10], [1025, 9, 1025]
+    else:
+        level, extents = [11, 3, 10], [2049, 9, 1025]
+    for name in ("hat", "biorthogonal"):
+        a = closed_random(rng, extents, level, dtype)
+        lmin = [0, 0, 0]
+        fwd = gpu_named(pw, a, level, lmin, name, True)
+        assert_same(fwd, oracle_named(a, level, lmin, name, True), f"{extents} {name} fwd")
+        back = gpu_named(pw, fwd, level, lmin, name, False)
+        assert_same(back, oracle_named(fwd, level, lmin, name, False), f"{extents} {name} inv")
+
+
 @pytest.mark.parametrize("path", ["fused", "lds_line", "per_pass"])
 def test_kernel_paths_bit_exact(pw, path):
     rng = np.random.default_rng(13)
