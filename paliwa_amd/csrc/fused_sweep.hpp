@@ -611,17 +611,36 @@ hipError_t launch_one(Kernel kernel, SweepParams const &p, size_t bytes,
   return hipGetLastError();
 }
 
-// Lines of 64 chunks keep 512-thread tiles by default (a wide tile halves the
-// tile count, which small grids cannot spare); a strided float sweep over many
-// lines (at least kWideTileLines: two rounds of 1024-thread tiles on 256 CUs)
-// takes the 1024-thread ladder instead, where its doubled tile fits in LDS
-// (float64 lines of 1024 points, float32 lines of 2048): tile rows of 128
-// instead of 64 bytes.
+// The width of a strided float tile whose 512-thread rows are shorter than 256
+// bytes is chosen at launch where its 1024-thread form fits in LDS: 1024
+// threads (rows twice as long) over many lines (at least kWideTileLines: two
+// rounds of wide tiles on 256 CUs), 512 below, where halving the tile count
+// would leave CUs idle.  Covers float64 lines of 512 and 1024 points and
+// float32 lines of 512 to 2048.
 constexpr int64_t kWideTileLines = 8192;
 template <typename T, int K, int G, bool STRIDED> constexpr bool wide_ladder() {
-  return STRIDED && !kStridedDirect && G == 64 && !std::is_integral_v<T> &&
-         threads_for<T>(STRIDED, K, G) == 512 &&
+  return STRIDED && !kStridedDirect && G >= 4 && G <= 64 && !std::is_integral_v<T> &&
+         (512 / G) * int64_t(sizeof(T)) < 256 &&
          2 * (int64_t(K) * G + 1) * (512 / G) * int64_t(sizeof(T)) <= PALIWA_WIDE_TILE_LDS;
+}
+
+template <typename T, int K, int G, bool STRIDED, int NTO>
+hipError_t launch_ladder_nt(SweepParams const &p, int64_t lines, hipStream_t stream) {
+  using Geom = TileGeom<K, G, STRIDED, T, NTO>;
+  int64_t const tiles = (lines + Geom::TB - 1) / Geom::TB;
+  size_t const bytes = sizeof(T) * Geom::lds_elems(p.m, STRIDED);
+  bool const asc = p.schedule == kScheduleLadderAsc;
+  switch (p.pattern) {
+  case kPatOdd:
+    return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatOdd, true, NTO>, p, bytes, tiles, NTO, stream)
+               : launch_one(fused_ladder<T, K, G, STRIDED, kPatOdd, false, NTO>, p, bytes, tiles, NTO, stream);
+  case kPatOddEven:
+    return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatOddEven, true, NTO>, p, bytes, tiles, NTO, stream)
+               : launch_one(fused_ladder<T, K, G, STRIDED, kPatOddEven, false, NTO>, p, bytes, tiles, NTO, stream);
+  default:
+    return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatEvenOdd, true, NTO>, p, bytes, tiles, NTO, stream)
+               : launch_one(fused_ladder<T, K, G, STRIDED, kPatEvenOdd, false, NTO>, p, bytes, tiles, NTO, stream);
+  }
 }
 
 template <typename T, int K, int G, bool STRIDED>
@@ -630,25 +649,9 @@ hipError_t launch_kg_layout(SweepParams const &p, hipStream_t stream) {
   int64_t const lines = p.A * p.B;
   if constexpr (wide_ladder<T, K, G, STRIDED>()) {
     if ((p.schedule == kScheduleLadderAsc || p.schedule == kScheduleLadderDesc) &&
-        lines >= kWideTileLines) {
-      using Wide = TileGeom<K, G, STRIDED, T, 1024>;
-      int64_t const tiles = (lines + Wide::TB - 1) / Wide::TB;
-      size_t const bytes = sizeof(T) * Wide::lds_elems(p.m, STRIDED);
-      bool const asc = p.schedule == kScheduleLadderAsc;
-      switch (p.pattern) {
-      case kPatOdd:
-        return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatOdd, true, 1024>, p, bytes, tiles, 1024, stream)
-                   : launch_one(fused_ladder<T, K, G, STRIDED, kPatOdd, false, 1024>, p, bytes, tiles, 1024, stream);
-      case kPatOddEven:
-        return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatOddEven, true, 1024>, p, bytes, tiles, 1024, stream)
-                   : launch_one(fused_ladder<T, K, G, STRIDED, kPatOddEven, false, 1024>, p, bytes, tiles, 1024, stream);
-      case kPatEvenOdd:
-        return asc ? launch_one(fused_ladder<T, K, G, STRIDED, kPatEvenOdd, true, 1024>, p, bytes, tiles, 1024, stream)
-                   : launch_one(fused_ladder<T, K, G, STRIDED, kPatEvenOdd, false, 1024>, p, bytes, tiles, 1024, stream);
-      default:
-        break;
-      }
-    }
+        p.pattern != kPatNone)
+      return lines >= kWideTileLines ? launch_ladder_nt<T, K, G, STRIDED, 1024>(p, lines, stream)
+                                     : launch_ladder_nt<T, K, G, STRIDED, 512>(p, lines, stream);
   }
   int64_t const tiles = (lines + Geom::TB - 1) / Geom::TB;
   size_t const bytes = sizeof(T) * Geom::lds_elems(p.m, STRIDED);

@@ -128,9 +128,9 @@ inline int fused_chunk(int dtype, int L) {
 // are 256 bytes (257^3 float64: 53 us against 50.5 us), and the int32 ladder
 // keeps 512 (it would spill at 1024).  Lines of 64 chunks (G = 64) keep 512
 // here: a wide tile halves the tile count, and the 1025^2 float64 grid's
-// strided sweep (65 wide tiles for 256 CUs) took 11.5 instead of 8.2 us; the
-// single-grid launcher switches to 1024 threads for them at run time when the
-// sweep has enough lines (fused_sweep.hpp, wide_ladder).
+// strided sweep (65 wide tiles for 256 CUs) took 11.5 instead of 8.2 us.
+// This is the compiled default; the single-grid ladder launch picks either
+// width at run time from the line count (fused_sweep.hpp, wide_ladder).
 #ifndef PALIWA_STRIDED_THREADS
 #define PALIWA_STRIDED_THREADS 512
 #endif

@@ -129,16 +129,18 @@ def test_maximum_dimension_count(pw):
             assert_same(back, oracle_named(fwd, level, lmin, name, False), f"26-d {name} inv")
 
 
-@pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_wide_tiles_over_many_lines(pw, dtype):
-    """Strided sweeps over 9225 lines of 64 chunks (float64 period 1024,
-    float32 period 2048), which take the 1024-thread ladder: both directions
-    against the oracle."""
+@pytest.mark.parametrize("dtype,level,extents", [
+    (np.float64, [9, 4, 9], [513, 17, 513]),
+    (np.float64, [10, 3, 10], [1025, 9, 1025]),
+    (np.float32, [9, 4, 9], [513, 17, 513]),
+    (np.float32, [10, 3, 10], [1025, 9, 1025]),
+    (np.float32, [11, 3, 10], [2049, 9, 1025]),
+])
+def test_wide_tiles_over_many_lines(pw, dtype, level, extents):
+    """Strided sweeps over more than 8192 lines of 512 to 2048 points, which
+    take the 1024-thread ladder (the small-grid tests above run the same
+    periods with 512 threads): both directions against the oracle."""
     rng = np.random.default_rng(17)
-    if dtype == np.float64:
-        level, extents = [10, 3, 10], [1025, 9, 1025]
-    else:
-        level, extents = [11, 3, 10], [2049, 9, 1025]
     for name in ("hat", "biorthogonal"):
         a = closed_random(rng, extents, level, dtype)
         lmin = [0, 0, 0]
