@@ -615,9 +615,11 @@ hipError_t launch_one(Kernel kernel, SweepParams const &p, size_t bytes,
 // bytes is chosen at launch where its 1024-thread form fits in LDS: 1024
 // threads (rows twice as long) over many lines (at least kWideTileLines: two
 // rounds of wide tiles on 256 CUs), 512 below, where halving the tile count
-// would leave CUs idle.  Covers float64 lines of 512 and 1024 points and
-// float32 lines of 512 to 2048.
+// would leave CUs idle, and 256 where even 512-thread tiles would be fewer
+// than the CUs.  Covers float64 lines of 512 and 1024 points and float32
+// lines of 512 to 2048.
 constexpr int64_t kWideTileLines = 8192;
+constexpr int64_t kNarrowBelowTiles = 256; // CUs of the device
 template <typename T, int K, int G, bool STRIDED> constexpr bool wide_ladder() {
   return STRIDED && !kStridedDirect && G >= 4 && G <= 64 && !std::is_integral_v<T> &&
          (512 / G) * int64_t(sizeof(T)) < 256 &&
@@ -650,8 +652,14 @@ hipError_t launch_kg_layout(SweepParams const &p, hipStream_t stream) {
   if constexpr (wide_ladder<T, K, G, STRIDED>()) {
     if ((p.schedule == kScheduleLadderAsc || p.schedule == kScheduleLadderDesc) &&
         p.pattern != kPatNone)
-      return lines >= kWideTileLines ? launch_ladder_nt<T, K, G, STRIDED, 1024>(p, lines, stream)
-                                     : launch_ladder_nt<T, K, G, STRIDED, 512>(p, lines, stream);
+    {
+      if (lines >= kWideTileLines)
+        return launch_ladder_nt<T, K, G, STRIDED, 1024>(p, lines, stream);
+      // fewer 512-thread tiles than CUs (a small, cache-resident grid): halve them
+      if (lines * G < kNarrowBelowTiles * 512)
+        return launch_ladder_nt<T, K, G, STRIDED, 256>(p, lines, stream);
+      return launch_ladder_nt<T, K, G, STRIDED, 512>(p, lines, stream);
+    }
   }
   int64_t const tiles = (lines + Geom::TB - 1) / Geom::TB;
   size_t const bytes = sizeof(T) * Geom::lds_elems(p.m, STRIDED);
