@@ -620,7 +620,7 @@ hipError_t launch_one(Kernel kernel, SweepParams const &p, size_t bytes,
 // lines of 512 to 2048.
 constexpr int64_t kWideTileLines = 8192;
 constexpr int64_t kNarrowBelowTiles = 256; // CUs of the device
-template <typename T, int K, int G, bool STRIDED> constexpr bool wide_ladder() {
+template <typename T, int K, int G, bool STRIDED> constexpr bool width_at_launch() {
   return STRIDED && !kStridedDirect && G >= 4 && G <= 64 && !std::is_integral_v<T> &&
          (512 / G) * int64_t(sizeof(T)) < 256 &&
          2 * (int64_t(K) * G + 1) * (512 / G) * int64_t(sizeof(T)) <= PALIWA_WIDE_TILE_LDS;
@@ -649,10 +649,9 @@ template <typename T, int K, int G, bool STRIDED>
 hipError_t launch_kg_layout(SweepParams const &p, hipStream_t stream) {
   using Geom = TileGeom<K, G, STRIDED, T>;
   int64_t const lines = p.A * p.B;
-  if constexpr (wide_ladder<T, K, G, STRIDED>()) {
+  if constexpr (width_at_launch<T, K, G, STRIDED>()) {
     if ((p.schedule == kScheduleLadderAsc || p.schedule == kScheduleLadderDesc) &&
-        p.pattern != kPatNone)
-    {
+        p.pattern != kPatNone) {
       if (lines >= kWideTileLines)
         return launch_ladder_nt<T, K, G, STRIDED, 1024>(p, lines, stream);
       // fewer 512-thread tiles than CUs (a small, cache-resident grid): halve them

@@ -130,7 +130,7 @@ inline int fused_chunk(int dtype, int L) {
 // here: a wide tile halves the tile count, and the 1025^2 float64 grid's
 // strided sweep (65 wide tiles for 256 CUs) took 11.5 instead of 8.2 us.
 // This is the compiled default; the single-grid ladder launch picks either
-// width at run time from the line count (fused_sweep.hpp, wide_ladder).
+// width at run time from the line count (fused_sweep.hpp, width_at_launch).
 #ifndef PALIWA_STRIDED_THREADS
 #define PALIWA_STRIDED_THREADS 512
 #endif
