@@ -36,6 +36,7 @@ struct paliwa_batch {
     bool fused = false;
     bool lds_lines = false;          // longer lines: one batched LDS-line launch
     int64_t tiles = 0;               // fused groups: tiles; LDS-line groups: lines
+    int32_t threads = 0;             // fused strided groups: tile width (0: default)
     BatchGrid *grids_dev = nullptr;  // fused groups: descriptors on the device
     int32_t *tile_grid_dev = nullptr;
     std::vector<int32_t> members;    // grid indices
@@ -67,6 +68,7 @@ int launch_group(paliwa_batch &b, paliwa_batch::Group const &grp,
     SweepParams p{};
     p.m = grp.m;
     p.L = grp.L;
+    p.threads = grp.threads;
     set_filters(p, filters, nfilters);
     set_named_levels(p, grp.L, b.lmin[grp.dim], forward);
     classify(p);
@@ -290,9 +292,22 @@ int paliwa_batch_create(paliwa_batch **plan, int32_t dtype, int32_t ndim,
       grp.lds_lines = !grp.fused && (int64_t(1) << grp.L) <= lds_line_max_period(dtype);
       std::vector<BatchGrid> gd;
       std::vector<int32_t> tg;
+      if (grp.fused && grp.strided) {
+        // the tile width of a strided float ladder follows the group's lines
+        int64_t lines = 0;
+        for (int32_t g : members) {
+          SweepParams const q = make_params(b->data[g], ndim, b->extents_of(g), b->level_of(g), d);
+          lines += q.A * q.B;
+        }
+        int const K = fused_chunk(dtype, grp.L);
+        grp.threads = ladder_threads_for(K, (1 << grp.L) / K, dtype == PALIWA_FLOAT64 ? 8 : 4, lines);
+      }
       if (grp.fused || grp.lds_lines) {
-        // fused: tiles of fused_tile_lines lines; LDS-line: one line per tile
-        int64_t const tb = grp.fused ? fused_tile_lines(dtype, grp.L, grp.strided) : 1;
+        // fused: tiles of fused_tile_lines lines (or threads / G); LDS-line:
+        // one line per tile
+        int64_t const tb = !grp.fused ? 1
+                           : grp.threads ? grp.threads / ((1 << grp.L) / fused_chunk(dtype, grp.L))
+                                         : fused_tile_lines(dtype, grp.L, grp.strided);
         for (size_t k = 0; k < members.size(); ++k) {
           int32_t const g = members[k];
           SweepParams const p =

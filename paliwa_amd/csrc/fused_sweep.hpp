@@ -567,11 +567,11 @@ __global__ __launch_bounds__(NTO ? NTO : threads_for<T>(STRIDED, K, G)) void fus
 
 // The same sweep over the tiles of many grids with equal line length along
 // the axis (one launch of a batched transform).
-template <typename T, int K, int G, bool STRIDED, int PAT, bool ASC>
-__global__ __launch_bounds__(threads_for<T>(STRIDED, K, G)) void fused_ladder_batched(
+template <typename T, int K, int G, bool STRIDED, int PAT, bool ASC, int NTO = 0>
+__global__ __launch_bounds__(NTO ? NTO : threads_for<T>(STRIDED, K, G)) void fused_ladder_batched(
     SweepParams p, uint32_t magic_m, int64_t ntiles, ManyGrids grids) {
   extern __shared__ unsigned char smem_raw[];
-  sweep_tiles<T, K, G, STRIDED, 0, PAT, ASC, 0>(p, grids, magic_m, ntiles,
+  sweep_tiles<T, K, G, STRIDED, 0, PAT, ASC, NTO>(p, grids, magic_m, ntiles,
                                              reinterpret_cast<T *>(smem_raw));
 }
 
@@ -612,18 +612,18 @@ hipError_t launch_one(Kernel kernel, SweepParams const &p, size_t bytes,
 }
 
 // The width of a strided float tile whose 512-thread rows are shorter than 256
-// bytes is chosen at launch where its 1024-thread form fits in LDS: 1024
-// threads (rows twice as long) over many lines (at least kWideTileLines: two
-// rounds of wide tiles on 256 CUs), 512 below, where halving the tile count
-// would leave CUs idle, and 256 where even 512-thread tiles would be fewer
-// than the CUs.  Covers float64 lines of 512 and 1024 points and float32
-// lines of 512 to 2048.
-constexpr int64_t kWideTileLines = 8192;
-constexpr int64_t kNarrowBelowTiles = 256; // CUs of the device
+// bytes (float64 lines of 512 points and up, float32 of 512 and up) is chosen
+// at launch from the line count (ladder_threads_for): the single-grid
+// launcher from the grid's lines, a batched plan per group from the group's.
 template <typename T, int K, int G, bool STRIDED> constexpr bool width_at_launch() {
   return STRIDED && !kStridedDirect && G >= 4 && G <= 64 && !std::is_integral_v<T> &&
-         (512 / G) * int64_t(sizeof(T)) < 256 &&
-         2 * (int64_t(K) * G + 1) * (512 / G) * int64_t(sizeof(T)) <= PALIWA_WIDE_TILE_LDS;
+         threads_for<T>(STRIDED, K, G) <= 1024 && (512 / G) * int64_t(sizeof(T)) < 256;
+}
+template <typename T, int K, int G> constexpr bool wide_fits() {
+  return 2 * (int64_t(K) * G + 1) * (512 / G) * int64_t(sizeof(T)) <= PALIWA_WIDE_TILE_LDS;
+}
+template <typename T, int K, int G> constexpr int ladder_threads(int64_t lines) {
+  return width_at_launch<T, K, G, true>() ? ladder_threads_for(K, G, int(sizeof(T)), lines) : 0;
 }
 
 template <typename T, int K, int G, bool STRIDED, int NTO>
@@ -652,12 +652,12 @@ hipError_t launch_kg_layout(SweepParams const &p, hipStream_t stream) {
   if constexpr (width_at_launch<T, K, G, STRIDED>()) {
     if ((p.schedule == kScheduleLadderAsc || p.schedule == kScheduleLadderDesc) &&
         p.pattern != kPatNone) {
-      if (lines >= kWideTileLines)
-        return launch_ladder_nt<T, K, G, STRIDED, 1024>(p, lines, stream);
-      // fewer 512-thread tiles than CUs (a small, cache-resident grid): halve them
-      if (lines * G < kNarrowBelowTiles * 512)
-        return launch_ladder_nt<T, K, G, STRIDED, 256>(p, lines, stream);
-      return launch_ladder_nt<T, K, G, STRIDED, 512>(p, lines, stream);
+      int const nt = ladder_threads<T, K, G>(lines);
+      if constexpr (wide_fits<T, K, G>())
+        if (nt == 1024)
+          return launch_ladder_nt<T, K, G, STRIDED, 1024>(p, lines, stream);
+      return nt == 256 ? launch_ladder_nt<T, K, G, STRIDED, 256>(p, lines, stream)
+                       : launch_ladder_nt<T, K, G, STRIDED, 512>(p, lines, stream);
     }
   }
   int64_t const tiles = (lines + Geom::TB - 1) / Geom::TB;
@@ -689,27 +689,46 @@ hipError_t launch_kg(SweepParams const &p, hipStream_t stream) {
 
 // Batched launches exist for the ladder schedules only (the named
 // hierarchize / dehierarchize transforms).
-template <typename T, int K, int G, bool STRIDED>
-hipError_t launch_batched_layout(SweepParams const &p, ManyGrids grids,
-                                 int64_t tiles, hipStream_t stream) {
-  using Geom = TileGeom<K, G, STRIDED, T>;
+// `tiles` tiles of p.threads threads (the compiled default when 0; the plan
+// built its tile table with the same width).
+template <typename T, int K, int G, bool STRIDED, int NTO>
+hipError_t launch_batched_nt(SweepParams const &p, ManyGrids grids, int64_t tiles,
+                             hipStream_t stream) {
+  using Geom = TileGeom<K, G, STRIDED, T, NTO>;
   size_t const bytes = sizeof(T) * Geom::lds_elems(p.m, STRIDED);
   bool const asc = p.schedule == kScheduleLadderAsc;
-  if (p.schedule != kScheduleLadderAsc && p.schedule != kScheduleLadderDesc)
-    return hipErrorInvalidValue;
   switch (p.pattern) {
   case kPatOdd:
-    return asc ? launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOdd, true>, p, bytes, tiles, Geom::NT, stream, grids)
-               : launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOdd, false>, p, bytes, tiles, Geom::NT, stream, grids);
+    return asc ? launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOdd, true, NTO>, p, bytes, tiles, Geom::NT, stream, grids)
+               : launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOdd, false, NTO>, p, bytes, tiles, Geom::NT, stream, grids);
   case kPatOddEven:
-    return asc ? launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOddEven, true>, p, bytes, tiles, Geom::NT, stream, grids)
-               : launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOddEven, false>, p, bytes, tiles, Geom::NT, stream, grids);
+    return asc ? launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOddEven, true, NTO>, p, bytes, tiles, Geom::NT, stream, grids)
+               : launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatOddEven, false, NTO>, p, bytes, tiles, Geom::NT, stream, grids);
   case kPatEvenOdd:
-    return asc ? launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatEvenOdd, true>, p, bytes, tiles, Geom::NT, stream, grids)
-               : launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatEvenOdd, false>, p, bytes, tiles, Geom::NT, stream, grids);
+    return asc ? launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatEvenOdd, true, NTO>, p, bytes, tiles, Geom::NT, stream, grids)
+               : launch_one(fused_ladder_batched<T, K, G, STRIDED, kPatEvenOdd, false, NTO>, p, bytes, tiles, Geom::NT, stream, grids);
   default:
     return hipErrorInvalidValue;
   }
+}
+
+template <typename T, int K, int G, bool STRIDED>
+hipError_t launch_batched_layout(SweepParams const &p, ManyGrids grids,
+                                 int64_t tiles, hipStream_t stream) {
+  if (p.schedule != kScheduleLadderAsc && p.schedule != kScheduleLadderDesc)
+    return hipErrorInvalidValue;
+  if constexpr (width_at_launch<T, K, G, STRIDED>()) {
+    if constexpr (wide_fits<T, K, G>())
+      if (p.threads == 1024)
+        return launch_batched_nt<T, K, G, STRIDED, 1024>(p, grids, tiles, stream);
+    if (p.threads == 256)
+      return launch_batched_nt<T, K, G, STRIDED, 256>(p, grids, tiles, stream);
+    if (p.threads == 512)
+      return launch_batched_nt<T, K, G, STRIDED, 512>(p, grids, tiles, stream);
+  }
+  if (p.threads != 0 && p.threads != TileGeom<K, G, STRIDED, T>::NT)
+    return hipErrorInvalidValue; // a tile table built for another width
+  return launch_batched_nt<T, K, G, STRIDED, 0>(p, grids, tiles, stream);
 }
 
 template <typename T, int K, int G>

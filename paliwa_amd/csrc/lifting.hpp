@@ -33,6 +33,9 @@ struct SweepParams {
   int32_t lo, hi;   // log2 stride range (ladder schedules only)
   // int32 sweeps only: set to 1 when a result left the int32 range (or null)
   int32_t *overflow;
+  // batched fused launches: workgroup size of the group's tiles chosen by the
+  // plan from its line count (ladder_threads_for), 0 for the compiled default
+  int32_t threads;
 };
 
 enum Schedule : int32_t {
@@ -153,6 +156,20 @@ constexpr int fused_threads(bool strided, int K, int G, int elem_bytes,
   long const rows = 512 / G;                                   // tile width at 512
   long const wide = 2 * (long(K) * G + 1) * rows * elem_bytes; // LDS at 1024
   return rows * elem_bytes < 256 && wide <= PALIWA_WIDE_TILE_LDS ? 1024 : base;
+}
+// Workgroup size of a strided float ladder over `lines` lines whose 512-thread
+// tile rows are shorter than 256 bytes (4 <= G <= 64); 0 for the others (the
+// compiled default).  1024 threads (rows twice as long) over at least 8192
+// lines, two rounds of wide tiles on 256 CUs, where that tile fits in LDS; 256
+// where even 512-thread tiles would be fewer than the 256 CUs (a small,
+// cache-resident grid); 512 otherwise.
+constexpr int ladder_threads_for(int K, int G, int elem_bytes, long lines) {
+  if (G < 4 || G > 64 || (512 / G) * elem_bytes >= 256)
+    return 0;
+  bool const wide = 2 * (long(K) * G + 1) * (512 / G) * elem_bytes <= PALIWA_WIDE_TILE_LDS;
+  if (wide && lines >= 8192)
+    return 1024;
+  return lines * G < 256L * 512 ? 256 : 512;
 }
 // Lines per tile of the fused kernel.
 inline int fused_tile_lines(int dtype, int L, bool strided) {
