@@ -132,8 +132,9 @@ inline int fused_chunk(int dtype, int L) {
 // keeps 512 (it would spill at 1024).  Lines of 64 chunks (G = 64) keep 512
 // here: a wide tile halves the tile count, and the 1025^2 float64 grid's
 // strided sweep (65 wide tiles for 256 CUs) took 11.5 instead of 8.2 us.
-// This is the compiled default; the single-grid ladder launch picks either
-// width at run time from the line count (fused_sweep.hpp, width_at_launch).
+// This is the compiled default; strided float ladders choose their width at
+// launch from the line count (ladder_threads_for below: the single-grid launch
+// from the grid's lines, a batched plan per group).
 #ifndef PALIWA_STRIDED_THREADS
 #define PALIWA_STRIDED_THREADS 512
 #endif
