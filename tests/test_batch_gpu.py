@@ -116,6 +116,34 @@ def test_batch_wide_tiles(pw, dtype):
         batch.close()
 
 
+@pytest.mark.parametrize("dtype,levels", [
+    (np.float64, [(9, 4, 9), (9, 5, 8), (4, 9, 9)]),
+    (np.float32, [(10, 3, 10), (10, 4, 8), (3, 10, 10)]),
+])
+def test_batch_groups_over_many_lines(pw, dtype, levels):
+    """A strided group of more than 8192 lines of 512 (float64) or 1024
+    (float32) points, whose plan builds its tile table for 1024-thread tiles:
+    every grid against the oracle, both directions."""
+    lmin, lmax = [0, 0, 0], [10, 10, 10]
+    arrays = make_grids(levels, True, dtype, seed=25)
+    grids = [torch.from_numpy(a).cuda() for a in arrays]
+    batch = pw.CombinationBatch(grids, levels, lmin, lmax)
+    batch.hierarchize("biorthogonal")
+    torch.cuda.synchronize()
+    wants = []
+    for a, g, l in zip(arrays, grids, levels):
+        want = a.copy()
+        ob.hierarchize(want, list(l), lmin, lmax, "biorthogonal")
+        wants.append(want)
+        assert np.array_equal(g.cpu().numpy(), want), l
+    batch.dehierarchize("biorthogonal")
+    torch.cuda.synchronize()
+    for want, g, l in zip(wants, grids, levels):
+        ob.dehierarchize(want, list(l), lmin, lmax, "biorthogonal")
+        assert np.array_equal(g.cpu().numpy(), want), l
+    batch.close()
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_batch_long_lines(pw, dtype):
     """Lines longer than the fused kernel holds (2^12 and 2^13 float64, 2^13
