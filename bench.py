@@ -178,6 +178,15 @@ def cpu_baseline(grid_cpu, budget_s):
         detail.append(f"{label} {reps} reps {dt:.3f} s/rep {gbs:.2f} GB/s")
         if best is None or gbs > best:
             best = gbs
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {
         "value": round(best, 3),
         "unit": "GB/s",
@@ -185,7 +194,7 @@ def cpu_baseline(grid_cpu, budget_s):
         "kind": "port",
         "sample": "257^3 biorthogonal float64 forward+inverse, C++/OpenMP "
                   "restatement of the reference's CPU schedules (oracle/), "
-                  + "; ".join(detail),
+                  f"{ob.num_threads()} threads on {model}: " + "; ".join(detail),
     }
 
 
