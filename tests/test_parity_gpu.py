@@ -243,6 +243,29 @@ def test_long_lines_lds_and_per_pass(pw):
             assert_same(got, oracle_named(a, level, lmin, name, True), f"L={L} {extents}")
 
 
+def test_axes_without_levels(pw):
+    """Axes whose minimum level equals their level sweep nothing: a grid
+    with every axis so, or of level 0 only, comes back unchanged, and mixed
+    grids match the oracle (which transforms only the other axes)."""
+    rng = np.random.default_rng(17)
+    cases = [
+        ([5, 4], [33, 16], [5, 4]),          # no axis sweeps
+        ([0, 0, 0], [2, 1, 2], [0, 0, 0]),   # level 0 everywhere
+        ([5, 4, 3], [33, 16, 9], [5, 0, 3]),  # only the middle axis sweeps
+        ([6, 3], [64, 9], [2, 3]),           # only the first axis sweeps
+    ]
+    for name in WAVELETS:
+        for dtype in (np.float64, np.float32):
+            for level, extents, lmin in cases:
+                a = closed_random(rng, extents, level, dtype)
+                for fwd in (True, False):
+                    got = gpu_named(pw, a, level, lmin, name, fwd)
+                    want = oracle_named(a, level, lmin, name, fwd)
+                    assert_same(got, want, f"{extents} lmin={lmin} {name} fwd={fwd}")
+                    if lmin == level:
+                        assert np.array_equal(got, a)
+
+
 def test_python_api_errors_on_gpu(pw):
     t = torch.zeros(17, 16, dtype=torch.float64, device="cuda")
     with pytest.raises(KeyError):
