@@ -518,10 +518,12 @@ def main():
 
     # clock settle (untimed), then the warmup steps
     settle_t0 = time.perf_counter()
+    settle_rounds = 0
     while (time.perf_counter() - settle_t0) * 1e3 < args.settle_ms:
         for _ in range(10):
             step()
         torch.cuda.synchronize()
+        settle_rounds += 1
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -591,9 +593,19 @@ def main():
     stream_ms = c0.elapsed_time(c1) / 50
     del scratch
 
-    # round-trip check of the data the benchmark transformed
+    # round-trip error after every transform above (warmup, timed and
+    # per-launch steps, the cold sweeps), and after one forward + inverse of
+    # the original data
     torch.cuda.synchronize()
-    rt_err = (grid - original).abs().max().item() / original.abs().max().item()
+    scale = original.abs().max().item()
+    rt_err = (grid - original).abs().max().item() / scale
+    once = original.clone()
+    pw.hierarchize(once, level, lmin, level, WAVELET)
+    pw.dehierarchize(once, level, lmin, level, WAVELET)
+    torch.cuda.synchronize()
+    rt_once = (once - original).abs().max().item() / scale
+    del once
+    n_round_trips = 10 * settle_rounds + args.warmup + 2 * args.steps + 6
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
@@ -628,7 +640,8 @@ def main():
             "cold_GBps": round(bytes_per_step / (cold_step_ms * 1e-3) / 1e9, 2),
             "per_kernel_ms": {f"{k[0]}_dim{k[1]}": round(v, 4) for k, v in sorted(avg.items())},
             "cold_per_kernel_ms": {f"{k[0]}_dim{k[1]}": round(v, 4) for k, v in sorted(cold.items())},
-            "round_trip_rel_err": rt_err,
+            "round_trip_rel_err": rt_once,
+            "accumulated_rel_err": {"round_trips": n_round_trips, "max_rel": rt_err},
         },
         "roofline": {
             "bound": "hbm",
