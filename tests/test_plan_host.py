@@ -94,3 +94,41 @@ def test_planner_output_form():
     assert r == [(-1, -1), (8, 8), (4, 12), (6, 10), (5, 11)]
     w, r = partial.transform_bounds((0, 15), 4, 4, [4, 3, 2, 1], [(1, (0.0, 1.0, 0.0))])
     assert w[0] == [(1, 15)] and w[3] == [(8, 8)]
+
+
+def test_error_precedence():
+    """Walking backwards, the reference meets the last level's range check
+    before any pass's parity check (paliwa/paliwa_domains.hpp:407-414)."""
+    bad = [(1, (0.0, 1.0, 0.0)), (2, (0.0, 1.0, 0.0))]
+    with pytest.raises(Exception, match="Filter offset not supported"):
+        partial.transform_bounds((0, 0), 4, 4, [5, 4], bad)
+    with pytest.raises(Exception, match="Invalid transform level"):
+        partial.transform_bounds((0, 0), 4, 4, [4, 5], bad)
+    with pytest.raises(Exception, match="consecutive"):
+        partial.transform_bounds((0, 0), 4, 4, [4, 2], bad)
+    # no levels: no pass runs, so no parity is inspected
+    w, r = partial.transform_bounds((3, 5), 4, 4, [], bad)
+    assert w == [] and r[0] == (-1, -1)
+
+
+def test_cover_ties_leave_out_the_wrapping_gap():
+    assert partial.covering_periodic_bounds([0, 32], 64) == (0, 32)
+    # two equal interior gaps: the first one is left out
+    assert partial.covering_periodic_bounds([0, 10, 20, 21], 30) == (10, 0)
+    assert partial.covering_periodic_bounds([5, 6], 8) == (5, 6)
+
+
+def test_largest_period_saturates():
+    """At the largest period (2^62) a deep multi-pass ladder makes every
+    level's dependency cone cover its whole lattice; the runs must not
+    overflow on the way.  Closed form: a whole lattice of level h >= 1 is
+    {2^(62-h), 2^62 - 2^(62-h)}, level 0 is {0, 0}."""
+    offsets = [1, 0] * 8
+    levels = list(range(62, 0, -1))
+    filters = [(o, (0.0, 1.0, 0.0)) for o in offsets]
+    w, r = partial.transform_bounds((0, 0), 62, 62, levels, filters)
+    assert r[0] == (0, 0)
+    for h in range(1, 63):
+        assert r[h] == (1 << (62 - h), (1 << 62) - (1 << (62 - h))), h
+    # the coarsest level's passes write its whole lattice
+    assert w[-1][0] == (1 << 61, 1 << 61) and w[-1][1] == (0, 0)
