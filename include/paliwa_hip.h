@@ -54,7 +54,12 @@ extern "C" {
                                   paliwa/paliwa_dimensions.hpp:10-69 */
 #define PALIWA_MAX_FILTERS 16  /* passes per filter list */
 #define PALIWA_MAX_LEVELS 64   /* entries of a level list */
-#define PALIWA_MAX_LEVEL 40    /* per-axis level */
+#define PALIWA_MAX_LEVEL 40    /* maximum_level of an axis; sizes the
+                                  learned transform's statistics */
+#define PALIWA_MAX_AXIS_LEVEL 30 /* stored level of an axis: an extent is at
+                                    most 2^30 + 1 points (lines are indexed
+                                    with 32-bit integers); above it a call
+                                    returns PALIWA_ERR_UNSUPPORTED */
 
 /* Kernel schedules for paliwa_transform_in_path. */
 #define PALIWA_PATH_AUTO 0     /* best available (fused when it fits) */

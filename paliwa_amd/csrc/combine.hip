@@ -43,10 +43,7 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kLogBlock = 10;
 constexpr int64_t kBlockPositions = int64_t(1) << kLogBlock;
-#ifndef PALIWA_COMBINE_UNROLL
-#define PALIWA_COMBINE_UNROLL 8
-#endif
-constexpr int kUnroll = PALIWA_COMBINE_UNROLL; // grids per step
+constexpr int kUnroll = 8;                      // grids per step
 constexpr int kChunk = 64;                      // term records per chunk
 constexpr int kSteps = kChunk / kUnroll;        // steps per chunk
 static_assert(kChunk % kUnroll == 0, "a step never straddles two chunks");

@@ -57,6 +57,10 @@ int check_grid(void const *data, int32_t dtype, int32_t ndim,
     if (level[d] < 0 || level[d] > PALIWA_MAX_LEVEL)
       return fail(PALIWA_ERR_LEVEL, "level[" + std::to_string(d) +
                                         "] out of range [0, 40]");
+    if (level[d] > PALIWA_MAX_AXIS_LEVEL)
+      return fail(PALIWA_ERR_UNSUPPORTED,
+                  "level[" + std::to_string(d) +
+                      "] above 30: extents of 2^31 points or more are not supported");
     if (maximum_level != nullptr && level[d] > maximum_level[d])
       return fail(PALIWA_ERR_LEVEL, "level[" + std::to_string(d) +
                                         "] exceeds maximum_level");

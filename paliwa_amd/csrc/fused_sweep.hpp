@@ -44,11 +44,6 @@ namespace fused {
 
 constexpr int kThreads = 256;
 constexpr int kXcds = 8;
-#ifdef PALIWA_STRIDED_DIRECT
-constexpr bool kStridedDirect = true;
-#else
-constexpr bool kStridedDirect = false;
-#endif
 // Workgroup size (see fused_threads): along a strided axis a tile holds NT/G
 // columns and every row of the tile is one run of NT/G consecutive points in
 // memory; wider runs waste less of the cache lines at their ends (rows start
@@ -250,7 +245,7 @@ template <int K, int G, bool STRIDED, typename T, int NTO = 0> struct TileGeom {
   static constexpr int PAD = G == 1 ? 0 : (128 / (ES * G) < 1 ? 1 : 128 / (ES * G));
   __host__ __device__ static int line_pitch(int m) { return (m + G * PAD) | 1; }
   __host__ __device__ static int lds_elems(int m, bool strided) {
-    if (strided && (G == 1 || kStridedDirect))
+    if (strided && G == 1)
       return 0; // columns go straight to registers
     return strided ? m * TB + G * PAD + 2 : TB * line_pitch(m);
   }
@@ -460,7 +455,7 @@ __device__ __forceinline__ void sweep_tiles(SweepParams const &p,
     __syncthreads();
   }
   int64_t t = xcd_tile(blockIdx.x, nwg);
-  if constexpr (STRIDED && (G == 1 || kStridedDirect)) {
+  if constexpr (STRIDED && G == 1) {
     // Lanes load their own chunks straight between HBM and registers: lane
     // (li, gi) holds rows gi*K .. gi*K+K-1 of column li of the tile.  With
     // one lane per line (G == 1) the TB lanes of a wavefront read TB
@@ -616,11 +611,11 @@ hipError_t launch_one(Kernel kernel, SweepParams const &p, size_t bytes,
 // at launch from the line count (ladder_threads_for): the single-grid
 // launcher from the grid's lines, a batched plan per group from the group's.
 template <typename T, int K, int G, bool STRIDED> constexpr bool width_at_launch() {
-  return STRIDED && !kStridedDirect && G >= 4 && G <= 64 && !std::is_integral_v<T> &&
+  return STRIDED && G >= 4 && G <= 64 && !std::is_integral_v<T> &&
          threads_for<T>(STRIDED, K, G) <= 1024 && (512 / G) * int64_t(sizeof(T)) < 256;
 }
 template <typename T, int K, int G> constexpr bool wide_fits() {
-  return 2 * (int64_t(K) * G + 1) * (512 / G) * int64_t(sizeof(T)) <= PALIWA_WIDE_TILE_LDS;
+  return 2 * (int64_t(K) * G + 1) * (512 / G) * int64_t(sizeof(T)) <= kWideTileLds;
 }
 template <typename T, int K, int G> constexpr int ladder_threads(int64_t lines) {
   return width_at_launch<T, K, G, true>() ? ladder_threads_for(K, G, int(sizeof(T)), lines) : 0;

@@ -109,11 +109,9 @@ hipError_t launch_fused_i32(SweepParams const &p, hipStream_t stream);
 
 // Points per lane (chunk length K) of the fused kernel for a line of period
 // 2^L; the kernel's tile holds 256*K / 2^L lines.
-#ifndef PALIWA_CHUNK_BYTES
-#define PALIWA_CHUNK_BYTES 128
-#endif
+constexpr int kChunkBytes = 128; // register chunk per lane
 constexpr int fused_chunk_of(int elem_bytes, int L) {
-  int const target = PALIWA_CHUNK_BYTES / elem_bytes;
+  int const target = kChunkBytes / elem_bytes;
   int const n = 1 << L;
   return n <= target ? n : (n / target <= 64 ? target : n / 64);
 }
@@ -135,28 +133,19 @@ inline int fused_chunk(int dtype, int L) {
 // This is the compiled default; strided float ladders choose their width at
 // launch from the line count (ladder_threads_for below: the single-grid launch
 // from the grid's lines, a batched plan per group).
-#ifndef PALIWA_STRIDED_THREADS
-#define PALIWA_STRIDED_THREADS 512
-#endif
-#ifndef PALIWA_STRIDED_THREADS_LONG
-#define PALIWA_STRIDED_THREADS_LONG PALIWA_STRIDED_THREADS
-#endif
-#ifndef PALIWA_WIDE_TILE_LDS
-#define PALIWA_WIDE_TILE_LDS (136 * 1024)
-#endif
-#ifndef PALIWA_CONTIG_THREADS
-#define PALIWA_CONTIG_THREADS 256
-#endif
+constexpr int kStridedThreads = 512;      // strided tiles of 4 or more lanes per line
+constexpr int kContigThreads = 256;       // contiguous tiles
+constexpr long kWideTileLds = 136 * 1024; // LDS a 1024-thread tile may use
 constexpr int fused_threads(bool strided, int K, int G, int elem_bytes,
                             bool integral = false) {
   if (!strided)
-    return G >= 4 ? PALIWA_CONTIG_THREADS : 256;
-  int const base = G >= 32 ? PALIWA_STRIDED_THREADS_LONG : (G >= 4 ? PALIWA_STRIDED_THREADS : 256);
+    return kContigThreads;
+  int const base = G >= 4 ? kStridedThreads : 256;
   if (integral || base != 512 || G > 32)
     return base;
   long const rows = 512 / G;                                   // tile width at 512
   long const wide = 2 * (long(K) * G + 1) * rows * elem_bytes; // LDS at 1024
-  return rows * elem_bytes < 256 && wide <= PALIWA_WIDE_TILE_LDS ? 1024 : base;
+  return rows * elem_bytes < 256 && wide <= kWideTileLds ? 1024 : base;
 }
 // Workgroup size of a strided float ladder over `lines` lines whose 512-thread
 // tile rows are shorter than 256 bytes (4 <= G <= 64); 0 for the others (the
@@ -167,7 +156,7 @@ constexpr int fused_threads(bool strided, int K, int G, int elem_bytes,
 constexpr int ladder_threads_for(int K, int G, int elem_bytes, long lines) {
   if (G < 4 || G > 64 || (512 / G) * elem_bytes >= 256)
     return 0;
-  bool const wide = 2 * (long(K) * G + 1) * (512 / G) * elem_bytes <= PALIWA_WIDE_TILE_LDS;
+  bool const wide = 2 * (long(K) * G + 1) * (512 / G) * elem_bytes <= kWideTileLds;
   if (wide && lines >= 8192)
     return 1024;
   return lines * G < 256L * 512 ? 256 : 512;

@@ -28,14 +28,12 @@
 #include <new>
 #include <vector>
 
-// The whole-period register form runs when SHARE * window >= period.
-#ifndef PALIWA_PARTIAL_FUSED_SHARE
-#define PALIWA_PARTIAL_FUSED_SHARE 2
-#endif
-
 namespace {
 
 using namespace paliwa_hip;
+
+// The whole-period register form runs when kFusedShare * window >= period.
+constexpr int64_t kFusedShare = 2;
 
 constexpr int kThreads = 256;
 constexpr int32_t kNoSource = std::numeric_limits<int32_t>::min();
@@ -577,7 +575,7 @@ int paliwa_partial_create(paliwa_partial **plan, int32_t dtype, int32_t ndim,
   // rows 0.024 against 0.011)
   std::vector<int16_t> row_source;
   if (B > 1 && level >= 1 && nlevels >= 1 && P <= fused_max_period(dtype) &&
-      PALIWA_PARTIAL_FUSED_SHARE * W >= P) {
+      kFusedShare * W >= P) {
     SweepParams sp{};
     sp.nlevels = nlevels;
     for (int32_t i = 0; i < nlevels; ++i)

@@ -193,3 +193,27 @@ def test_combination_levels():
     assert len(levels) == 220 + 286 + 364 + 455
     assert len(set(levels)) == len(levels)
     assert all(min(l) >= 1 and sum(l) in (13, 14, 15, 16) for l in levels)
+
+
+def test_axis_level_above_30_is_rejected_before_any_launch():
+    """Lines are indexed with 32-bit integers, so a stored extent of 2^31
+    points or more (a 2 x (2^31 + 1) grid is about 32 GB) is refused on the
+    host with PALIWA_ERR_UNSUPPORTED; level 30 still passes the level check
+    (it fails only on the fake pointer's shape here)."""
+    for lv in (31, 40):
+        ext = (2, (1 << lv) + 1)
+        assert call_named(_lib.lib.paliwa_hierarchize, extents=ext, level=(1, lv),
+                          lmax=(1, lv)) == _lib.PALIWA_ERR_UNSUPPORTED
+        assert "above 30" in _lib.last_error()
+        assert call_transform_in(levels=(lv,), dim=1, level=(1, lv),
+                                 extents=ext) == _lib.PALIWA_ERR_UNSUPPORTED
+    plan = ctypes.c_void_p(None)
+    ptrs = (ctypes.c_void_p * 1)(0x1000)
+    create = _lib.lib.paliwa_batch_create
+    assert create(ctypes.byref(plan), 0, 2, 1, ptrs, _lib.i64_array([2, (1 << 31) + 1]),
+                  _lib.i32_array([1, 31]), _lib.i32_array([0, 0]),
+                  _lib.i32_array([1, 31])) == _lib.PALIWA_ERR_UNSUPPORTED
+    assert plan.value is None
+    # level 30 is accepted by the level check; a wrong extent is reported
+    assert call_named(_lib.lib.paliwa_hierarchize, extents=(2, 7), level=(1, 30),
+                      lmax=(1, 30)) == _lib.PALIWA_ERR_SHAPE
