@@ -152,6 +152,76 @@ int paliwa_dehierarchize(void *data, int32_t dtype, int32_t ndim,
                          const int32_t *maximum_level,
                          const char *wavelet_name, void *stream);
 
+/* ---- Partial buffers ---------------------------------------------------- *
+ * The reference's transform_in takes a checked branch for every buffer that is
+ * not the complete strided lattice (paliwa/paliwa_transform.hpp:224-228,
+ * 241-248): detail::lifting_pass<CheckBounds = true> (:24-51) updates a point
+ * of the pass's half lattice only when it and both neighbours are stored
+ * (:31-32, 42-45).  A partial buffer is a dense row-major array over the
+ * positions of each axis; paliwa_axis_storage gives the finest-level
+ * coordinate (units of 2^-maximum_level[d]) of every position of axis d:
+ *   affine:  coordinates == NULL, coordinate(p) = front + p * step -- DDC's
+ *            DiscreteDomain (step 1) and StridedDiscreteDomain;
+ *   list:    coordinates = a device array of `count` ascending, distinct
+ *            coordinates -- DDC's SparseDiscreteDomain, e.g. local points
+ *            plus the ghosts of get_required_transform_domains
+ *            (paliwa/paliwa_domains.hpp:590-673).  Not read on the host.
+ * Every coordinate lies in [0, 2^maximum_level[d]); there are no periodic
+ * image planes.  A pass along `dim` at level l visits coordinates of parity
+ * offset * 2^(M-l) mod 2^(M-l+1) along dim (M = maximum_level[dim]) and
+ * multiples of 2^(maximum_level[d] - level[d]) along the other axes; other
+ * stored points keep their values.  A buffer that is the complete lattice
+ * of every axis (front 0, step 2^(maximum_level - level), count 2^level) runs
+ * the unchecked kernels, as the reference does.  Results are bit-identical to
+ * the reference's checked branch. */
+typedef struct paliwa_axis_storage {
+  int64_t count;              /* positions along the axis, at most 2^30 + 1 */
+  int64_t front;              /* affine: coordinate of position 0 (>= 0) */
+  int64_t step;               /* affine: spacing of the coordinates (>= 1) */
+  const int64_t *coordinates; /* list: device array, or NULL for affine */
+} paliwa_axis_storage;
+
+/* Replaces paliwa::transform_in<Dim> on a partial buffer
+ * (paliwa/paliwa_transform.hpp:215-251, CheckBounds branch). */
+int paliwa_transform_in_checked(void *data, int32_t dtype, int32_t ndim,
+                                const paliwa_axis_storage *storage,
+                                const int32_t *level,
+                                const int32_t *maximum_level, int32_t dim,
+                                const int32_t *levels, int32_t nlevels,
+                                const paliwa_filter *filters, int32_t nfilters,
+                                void *stream);
+
+/* Replace paliwa::hierarchize_in<Dim> / dehierarchize_in<Dim> on a partial
+ * buffer (paliwa/paliwa_transform.hpp:346-399). */
+int paliwa_hierarchize_in_checked(void *data, int32_t dtype, int32_t ndim,
+                                  const paliwa_axis_storage *storage,
+                                  const int32_t *level,
+                                  const int32_t *minimum_level,
+                                  const int32_t *maximum_level, int32_t dim,
+                                  const char *wavelet_name, void *stream);
+int paliwa_dehierarchize_in_checked(void *data, int32_t dtype, int32_t ndim,
+                                    const paliwa_axis_storage *storage,
+                                    const int32_t *level,
+                                    const int32_t *minimum_level,
+                                    const int32_t *maximum_level, int32_t dim,
+                                    const char *wavelet_name, void *stream);
+
+/* Replace paliwa::hierarchize / dehierarchize on a partial buffer
+ * (paliwa/paliwa_transform.hpp:401-431): axes 0..ndim-1, as in the test of
+ * test/test_paliwa_restrict.cpp:94-139. */
+int paliwa_hierarchize_checked(void *data, int32_t dtype, int32_t ndim,
+                               const paliwa_axis_storage *storage,
+                               const int32_t *level,
+                               const int32_t *minimum_level,
+                               const int32_t *maximum_level,
+                               const char *wavelet_name, void *stream);
+int paliwa_dehierarchize_checked(void *data, int32_t dtype, int32_t ndim,
+                                 const paliwa_axis_storage *storage,
+                                 const int32_t *level,
+                                 const int32_t *minimum_level,
+                                 const int32_t *maximum_level,
+                                 const char *wavelet_name, void *stream);
+
 /* ---- Batched transform of many grids (combination-technique sets) ------ */
 
 /* A plan over `ngrids` grids of one dtype and rank: grid g is data[g] with

@@ -227,6 +227,46 @@ int oracle_transform(void *data, int dtype, int ndim, int64_t const *extents,
   }
 }
 
+// Partial buffer: coords = the coordinates of every axis back to back
+// (counts[d] of them for axis d).
+int oracle_transform_in_checked(void *data, int dtype, int ndim,
+                                int64_t const *counts, int64_t const *coords,
+                                int32_t const *level,
+                                int32_t const *maximum_level, int dim,
+                                int32_t const *levels, int nlevels,
+                                int32_t const *offsets,
+                                double const *coefficients, int nfilters) {
+  try {
+    std::vector<std::vector<int64_t>> axes(ndim);
+    for (int d = 0; d < ndim; ++d) {
+      axes[d].assign(coords, coords + counts[d]);
+      coords += counts[d];
+    }
+    std::vector<int> lv(level, level + ndim), mx(maximum_level, maximum_level + ndim);
+    std::vector<int> ls(levels, levels + nlevels);
+    Filters filters;
+    for (int p = 0; p < nfilters; ++p)
+      filters.push_back({offsets[p],
+                         {coefficients[3 * p], coefficients[3 * p + 1],
+                          coefficients[3 * p + 2]}});
+    if (dtype == 0)
+      paliwa_oracle::transform_in_checked(static_cast<double *>(data), axes, lv, mx,
+                                          dim, ls, filters);
+    else if (dtype == 1)
+      paliwa_oracle::transform_in_checked(static_cast<float *>(data), axes, lv, mx,
+                                          dim, ls, filters);
+    else
+      throw std::invalid_argument("unknown dtype");
+    return 0;
+  } catch (std::out_of_range const &e) {
+    return fail(e, -2);
+  } catch (std::runtime_error const &e) {
+    return fail(e, -3);
+  } catch (std::exception const &e) {
+    return fail(e, -1);
+  }
+}
+
 int oracle_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

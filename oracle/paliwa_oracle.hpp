@@ -20,6 +20,8 @@
 //       paliwa/paliwa_domains.hpp:33-49, 85-111: at level l' of a grid stored
 //       at level L along an axis, the storage stride is 2^(L-l') and the
 //       period is 2^L.
+//   * the CheckBounds=true branch of lifting_pass for partial buffers
+//       (transform_in_checked; paliwa/paliwa_transform.hpp:31-32, 42-45)
 //   * PoleExecution::PackedPoles CPU schedule (gather a line, run every level
 //       and pass, scatter)  paliwa/paliwa_transform.hpp:119-184 -- used only as
 //       the CPU timing baseline; its arithmetic is the same.
@@ -37,7 +39,12 @@
 //   hierarchize/dehierarchize: round trip, tensor (Kronecker) structure
 //     checked against per-axis 1-D application, mass conservation and the
 //     lmin-lattice invariance of test/test_paliwa_transform.cpp:57-64, 127-131;
-//   packed-poles schedule: bit-identical to the direct schedule.
+//   packed-poles schedule: bit-identical to the direct schedule;
+//   transform_in_checked: equal to transform_in on the complete lattice, a
+//     brute force over a dictionary of stored points on tiny buffers, and
+//     the property of test/test_paliwa_restrict.cpp:94-139 (a buffer of the
+//     local points plus the planned required inputs reproduces the full
+//     transform on the local points, bit for bit).
 #pragma once
 
 #include <array>
@@ -204,6 +211,103 @@ void dehierarchize(T *g, std::vector<int64_t> const &extents,
   for (int d = 0; d < static_cast<int>(level.size()); ++d)
     transform_in(g, extents, level, d, increasing_levels(level[d], lmin[d]),
                  filters);
+}
+
+// ---- partial buffers: the CheckBounds=true branch -------------------------
+// paliwa/paliwa_transform.hpp:24-51 with CheckBounds, taken by transform_in
+// whenever the buffer is not the complete strided lattice (:224-228, 241-248).
+// A buffer is a dense row-major array over the positions of each axis; axis d
+// lists the finest-level coordinates of its positions (ascending, in
+// [0, 2^maximum_level[d])).  A pass along `dim` at level l (spacing
+// t = 2^(maximum_level[dim] - l), period P = 2^maximum_level[dim]) visits the
+// half lattice of parity `offset` -- coordinates offset*t + 2t*k along dim
+// and, along every other axis d, the multiples of 2^(maximum_level[d] -
+// level[d]) (the operating lattice of strided_domain_from_level,
+// paliwa/paliwa_domains.hpp:33-49, 85-111) -- and updates a point only when
+// it and both neighbours (x -+ t with the periodic wrap of :38-41) are stored
+// (:31-32, 42-45).  Written points and the points read have opposite parity,
+// so the order of visits does not matter.
+template <typename T>
+void transform_in_checked(T *g, std::vector<std::vector<int64_t>> const &coords,
+                          std::vector<int> const &level,
+                          std::vector<int> const &maximum_level, int dim,
+                          std::vector<int> const &levels, Filters const &filters) {
+  size_t const D = coords.size();
+  if (level.size() != D || maximum_level.size() != D || dim < 0 ||
+      dim >= static_cast<int>(D))
+    throw std::invalid_argument("bad rank or axis");
+  for (size_t d = 0; d < D; ++d) {
+    if (level[d] < 0 || level[d] > maximum_level[d] || maximum_level[d] > 62)
+      throw std::invalid_argument("require 0 <= level <= maximum_level <= 62");
+    for (size_t i = 0; i < coords[d].size(); ++i)
+      if (coords[d][i] < 0 || coords[d][i] >= (int64_t(1) << maximum_level[d]) ||
+          (i > 0 && coords[d][i] <= coords[d][i - 1]))
+        throw std::invalid_argument("coordinates must ascend inside the period");
+  }
+  // lines: every combination of positions of the other axes, in row-major
+  // order; a line takes part when each of those coordinates is on its
+  // axis's operating lattice
+  std::vector<int64_t> extent(D);
+  for (size_t d = 0; d < D; ++d)
+    extent[d] = static_cast<int64_t>(coords[d].size());
+  AxisView const v = axis_view(extent, dim);
+  if (v.A * v.m * v.B == 0)
+    return;
+  std::vector<char> line_on(v.A * v.B, 1);
+  for (int64_t line = 0; line < v.A * v.B; ++line) {
+    int64_t rest = line;
+    for (size_t d = D; d-- > 0;) {
+      if (static_cast<int>(d) == dim)
+        continue;
+      int64_t const pos = rest % extent[d];
+      rest /= extent[d];
+      int64_t const spacing = int64_t(1) << (maximum_level[d] - level[d]);
+      if (coords[d][pos] % spacing != 0)
+        line_on[line] = 0;
+    }
+  }
+  std::vector<int64_t> const &c = coords[dim];
+  auto position_of = [&c](int64_t x) -> int64_t { // -1: not stored
+    size_t lo = 0, hi = c.size();
+    while (lo < hi) {
+      size_t const mid = (lo + hi) / 2;
+      if (c[mid] < x)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    return lo < c.size() && c[lo] == x ? static_cast<int64_t>(lo) : -1;
+  };
+  int64_t const P = int64_t(1) << maximum_level[dim];
+  for (int l : levels) {
+    if (l < 1 || l > level[dim])
+      throw std::invalid_argument("operating level out of range");
+    int64_t const t = int64_t(1) << (maximum_level[dim] - l);
+    for (auto const &[offset, f] : filters) {
+      if (offset != 0 && offset != 1)
+        throw std::runtime_error("Filter offset not supported");
+      T const f0 = static_cast<T>(f[0]);
+      T const f1 = static_cast<T>(f[1]);
+      T const f2 = static_cast<T>(f[2]);
+      for (int64_t p = 0; p < v.m; ++p) {
+        int64_t const x = c[p];
+        if (x % (2 * t) != offset * t)
+          continue;
+        int64_t const lo = position_of(x == 0 ? P - t : x - t);
+        int64_t const hi = position_of(x + t == P ? 0 : x + t);
+        if (lo < 0 || hi < 0)
+          continue;
+        for (int64_t a = 0; a < v.A; ++a)
+          for (int64_t b = 0; b < v.B; ++b) {
+            if (!line_on[a * v.B + b])
+              continue;
+            T *block = g + a * v.m * v.B + b;
+            block[p * v.B] = f0 * block[lo * v.B] + f1 * block[p * v.B] +
+                             f2 * block[hi * v.B];
+          }
+      }
+    }
+  }
 }
 
 } // namespace paliwa_oracle

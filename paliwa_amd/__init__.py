@@ -6,6 +6,7 @@ include/paliwa_hip.h); this package is its Python host mirror.  Importing the
 package fails if the library has not been built.
 """
 from .transform import (  # noqa: F401
+    Affine,
     FilterOffsetError,
     HipError,
     LevelError,
@@ -14,12 +15,17 @@ from .transform import (  # noqa: F401
     UnknownWaveletError,
     UnsupportedError,
     dehierarchize,
+    dehierarchize_checked,
     dehierarchize_in,
+    dehierarchize_in_checked,
     hierarchize,
+    hierarchize_checked,
     hierarchize_in,
+    hierarchize_in_checked,
     lifting_wavelet_filter_offsets_and_coefficients,
     lifting_wavelet_reconstruct_offsets_and_coefficients,
     transform_in,
+    transform_in_checked,
     wavelet_filters,
 )
 from .batch import CombinationBatch, combination_levels  # noqa: F401

@@ -46,10 +46,16 @@ class paliwa_periodic_bounds(ctypes.Structure):
     _fields_ = [("min", ctypes.c_int64), ("max", ctypes.c_int64)]
 
 
+class paliwa_axis_storage(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_int64), ("front", ctypes.c_int64),
+                ("step", ctypes.c_int64), ("coordinates", ctypes.c_void_p)]
+
+
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
 _pbp = ctypes.POINTER(paliwa_periodic_bounds)
 _fp = ctypes.POINTER(paliwa_filter)
+_asp = ctypes.POINTER(paliwa_axis_storage)
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 
@@ -81,6 +87,26 @@ SIGNATURES = {
     "paliwa_dehierarchize": (
         ctypes.c_int,
         [_vp, _i32, _i32, _i64p, _i32p, _i32p, _i32p, ctypes.c_char_p, _vp],
+    ),
+    "paliwa_transform_in_checked": (
+        ctypes.c_int,
+        [_vp, _i32, _i32, _asp, _i32p, _i32p, _i32, _i32p, _i32, _fp, _i32, _vp],
+    ),
+    "paliwa_hierarchize_in_checked": (
+        ctypes.c_int,
+        [_vp, _i32, _i32, _asp, _i32p, _i32p, _i32p, _i32, ctypes.c_char_p, _vp],
+    ),
+    "paliwa_dehierarchize_in_checked": (
+        ctypes.c_int,
+        [_vp, _i32, _i32, _asp, _i32p, _i32p, _i32p, _i32, ctypes.c_char_p, _vp],
+    ),
+    "paliwa_hierarchize_checked": (
+        ctypes.c_int,
+        [_vp, _i32, _i32, _asp, _i32p, _i32p, _i32p, ctypes.c_char_p, _vp],
+    ),
+    "paliwa_dehierarchize_checked": (
+        ctypes.c_int,
+        [_vp, _i32, _i32, _asp, _i32p, _i32p, _i32p, ctypes.c_char_p, _vp],
     ),
     "paliwa_batch_create": (
         ctypes.c_int,

@@ -13,6 +13,21 @@ Same names, argument meaning, order and defaults as the C++ templates:
     transform_in(dim, grid, level, maximum_level, levels, filters)
         -- paliwa/paliwa_transform.hpp:215-251
 
+Partial buffers (the reference's checked branch, taken for every buffer that
+is not the complete strided lattice, paliwa/paliwa_transform.hpp:24-51,
+224-248) go through the same names with a ``storage`` argument: one entry
+per axis giving the finest-level coordinate of every position, either
+``Affine(front, step)`` (DDC's DiscreteDomain / StridedDiscreteDomain) or a
+1-D int64 tensor of ascending coordinates on the grid's device (DDC's
+SparseDiscreteDomain):
+
+    hierarchize_checked(grid, storage, level, minimum_level, maximum_level,
+                        wavelet_name="hat")
+    dehierarchize_checked(...), hierarchize_in_checked(dim, ...),
+    dehierarchize_in_checked(dim, ...),
+    transform_in_checked(dim, grid, storage, level, maximum_level, levels,
+                         filters)
+
 A grid is a contiguous float64/float32 torch tensor in GPU memory, row-major
 with the last axis contiguous, of extent 2^level[d] (or 2^level[d]+1 with a
 periodic image at the end) per axis.  The calls are asynchronous on the
@@ -47,6 +62,12 @@ __all__ = [
     "hierarchize_in",
     "dehierarchize_in",
     "transform_in",
+    "Affine",
+    "hierarchize_checked",
+    "dehierarchize_checked",
+    "hierarchize_in_checked",
+    "dehierarchize_in_checked",
+    "transform_in_checked",
     "wavelet_filters",
     "lifting_wavelet_filter_offsets_and_coefficients",
     "lifting_wavelet_reconstruct_offsets_and_coefficients",
@@ -227,6 +248,101 @@ def transform_in(dim: int, grid, level, maximum_level, levels: Iterable[int],
                 None if maximum_level is None else _lib.i32_array(maximum_level),
                 int(dim), _lib.i32_array(levels), len(levels),
                 _lib.filter_array(filters), len(filters), _lib.PATHS[path],
+                _stream_handle(grid, stream),
+            )
+        )
+    return True
+
+
+class Affine:
+    """Coordinates front + p * step along one axis of a partial buffer."""
+
+    def __init__(self, front: int, step: int = 1):
+        self.front = int(front)
+        self.step = int(step)
+
+
+def _storage_array(grid: torch.Tensor, storage):
+    storage = list(storage)
+    if len(storage) != grid.dim():
+        raise ShapeError(f"grid has {grid.dim()} axes but storage has {len(storage)}")
+    arr = (_lib.paliwa_axis_storage * len(storage))()
+    for d, s in enumerate(storage):
+        arr[d].count = int(grid.shape[d])
+        if isinstance(s, Affine):
+            arr[d].front, arr[d].step, arr[d].coordinates = s.front, s.step, None
+        else:
+            if not (isinstance(s, torch.Tensor) and s.dtype == torch.int64 and s.dim() == 1
+                    and s.is_contiguous() and s.device == grid.device):
+                raise ShapeError("a coordinate list must be a contiguous 1-D int64 tensor "
+                                 "on the grid's device")
+            if s.numel() != grid.shape[d]:
+                raise ShapeError(f"axis {d}: {s.numel()} coordinates for extent {grid.shape[d]}")
+            arr[d].front, arr[d].step = 0, 1
+            arr[d].coordinates = ctypes.c_void_p(s.data_ptr()) if s.numel() else None
+    return arr
+
+
+def _named_checked(fn, grid, storage, level, minimum_level, maximum_level,
+                   wavelet_name, stream, dim=None):
+    level = list(level)
+    _check_grid(grid, level)
+    args = [
+        ctypes.c_void_p(grid.data_ptr()), _dtype_code(grid), grid.dim(),
+        _storage_array(grid, storage), _lib.i32_array(level),
+        _lib.i32_array(minimum_level), _lib.i32_array(maximum_level),
+    ]
+    if dim is not None:
+        args.append(int(dim))
+    args += [str(wavelet_name).encode(), _stream_handle(grid, stream)]
+    with torch.cuda.device(grid.device):
+        check(fn(*args))
+
+
+def hierarchize_checked(grid, storage, level, minimum_level, maximum_level,
+                        wavelet_name: str = "hat", stream=None) -> None:
+    """hierarchize on a partial buffer (paliwa/paliwa_transform.hpp:401-415,
+    checked branch :24-51)."""
+    _named_checked(_lib.lib.paliwa_hierarchize_checked, grid, storage, level,
+                   minimum_level, maximum_level, wavelet_name, stream)
+
+
+def dehierarchize_checked(grid, storage, level, minimum_level, maximum_level,
+                          wavelet_name: str = "hat", stream=None) -> None:
+    """dehierarchize on a partial buffer (paliwa/paliwa_transform.hpp:417-431)."""
+    _named_checked(_lib.lib.paliwa_dehierarchize_checked, grid, storage, level,
+                   minimum_level, maximum_level, wavelet_name, stream)
+
+
+def hierarchize_in_checked(dim: int, grid, storage, level, minimum_level, maximum_level,
+                           wavelet_name: str = "hat", stream=None) -> bool:
+    _named_checked(_lib.lib.paliwa_hierarchize_in_checked, grid, storage, level,
+                   minimum_level, maximum_level, wavelet_name, stream, dim=dim)
+    return True
+
+
+def dehierarchize_in_checked(dim: int, grid, storage, level, minimum_level, maximum_level,
+                             wavelet_name: str = "hat", stream=None) -> bool:
+    _named_checked(_lib.lib.paliwa_dehierarchize_in_checked, grid, storage, level,
+                   minimum_level, maximum_level, wavelet_name, stream, dim=dim)
+    return True
+
+
+def transform_in_checked(dim: int, grid, storage, level, maximum_level,
+                         levels: Iterable[int], filters, stream=None) -> bool:
+    """transform_in on a partial buffer (paliwa/paliwa_transform.hpp:215-251,
+    checked branch)."""
+    level = list(level)
+    _check_grid(grid, level)
+    levels = [int(l) for l in levels]
+    filters = list(filters)
+    with torch.cuda.device(grid.device):
+        check(
+            _lib.lib.paliwa_transform_in_checked(
+                ctypes.c_void_p(grid.data_ptr()), _dtype_code(grid), grid.dim(),
+                _storage_array(grid, storage), _lib.i32_array(level),
+                _lib.i32_array(maximum_level), int(dim), _lib.i32_array(levels),
+                len(levels), _lib.filter_array(filters), len(filters),
                 _stream_handle(grid, stream),
             )
         )

@@ -45,6 +45,11 @@ def _load():
         ctypes.c_void_p, ctypes.c_int, ctypes.c_int, i64p, i32p, i32p, i32p,
         ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
     ]
+    lib.oracle_transform_in_checked.restype = ctypes.c_int
+    lib.oracle_transform_in_checked.argtypes = [
+        ctypes.c_void_p, ctypes.c_int, ctypes.c_int, i64p, i64p, i32p, i32p, ctypes.c_int,
+        i32p, ctypes.c_int, i32p, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+    ]
     lib.oracle_last_error.restype = ctypes.c_char_p
     lib.oracle_num_threads.restype = ctypes.c_int
     return lib
@@ -109,6 +114,35 @@ def transform_in(a: np.ndarray, level, dim: int, levels, filters,
     _check(lib.oracle_transform_in(
         a.ctypes.data, _dtype(a), a.ndim, _i64(a.shape), _i32(level), int(dim),
         _i32(levels), len(levels), _i32(offsets), carr, len(filters), schedule))
+    return a
+
+
+def transform_in_checked(a: np.ndarray, coords, level, maximum_level, dim: int, levels,
+                         filters) -> np.ndarray:
+    """The checked branch on a partial buffer: coords[d] = the ascending
+    finest-level coordinates of the positions of axis d."""
+    assert a.flags.c_contiguous and a.ndim == len(coords)
+    counts = [len(c) for c in coords]
+    assert list(a.shape) == counts
+    flat = [int(x) for c in coords for x in c]
+    offsets = [f[0] for f in filters]
+    coefs = [c for f in filters for c in f[1]]
+    carr = (ctypes.c_double * max(1, len(coefs)))(*coefs)
+    _check(lib.oracle_transform_in_checked(
+        a.ctypes.data, _dtype(a), a.ndim, _i64(counts), _i64(flat), _i32(level),
+        _i32(maximum_level), int(dim), _i32(levels), len(levels), _i32(offsets), carr,
+        len(filters)))
+    return a
+
+
+def transform_checked(a, coords, level, lmin, lmax, name, forward):
+    """hierarchize / dehierarchize of a partial buffer: axes 0..D-1, levels
+    decreasing (forward) or increasing, the named table."""
+    filters = (FORWARD if forward else RECONSTRUCT)[name]
+    for d in range(a.ndim):
+        levels = list(range(level[d], lmin[d], -1)) if forward else \
+            list(range(lmin[d] + 1, level[d] + 1))
+        transform_in_checked(a, coords, level, lmax, d, levels, filters)
     return a
 
 

@@ -217,3 +217,44 @@ def test_axis_level_above_30_is_rejected_before_any_launch():
     # level 30 is accepted by the level check; a wrong extent is reported
     assert call_named(_lib.lib.paliwa_hierarchize, extents=(2, 7), level=(1, 30),
                       lmax=(1, 30)) == _lib.PALIWA_ERR_SHAPE
+
+
+def test_checked_storage_validation():
+    """Partial-buffer entry points validate their storage on the host."""
+    S = _lib.paliwa_axis_storage
+
+    def storage(*axes):
+        arr = (S * len(axes))()
+        for i, (count, front, step, coords) in enumerate(axes):
+            arr[i].count, arr[i].front, arr[i].step, arr[i].coordinates = count, front, step, coords
+        return arr
+
+    def call(st, level=(3, 3), lmax=(4, 4), lmin=(0, 0), name=b"hat", ndim=2, dtype=0):
+        return _lib.lib.paliwa_hierarchize_checked(
+            FAKE, dtype, ndim, st, _lib.i32_array(level), _lib.i32_array(lmin),
+            _lib.i32_array(lmax), name, None)
+
+    ok_axis = (8, 0, 2, None)
+    assert call(storage(ok_axis, (16, 0, 0, None))) == _lib.PALIWA_ERR_INVALID_ARGUMENT  # step 0
+    assert call(storage(ok_axis, (16, -1, 1, None))) == _lib.PALIWA_ERR_INVALID_ARGUMENT  # front < 0
+    assert call(storage(ok_axis, (17, 0, 1, None))) == _lib.PALIWA_ERR_SHAPE  # past the period
+    assert call(storage(ok_axis, (8, 10, 1, None))) == _lib.PALIWA_ERR_SHAPE
+    assert call(storage(ok_axis, ((1 << 30) + 2, 0, 1, 0x1000))) == _lib.PALIWA_ERR_UNSUPPORTED
+    assert call(storage(ok_axis, ok_axis), level=(3, 5)) == _lib.PALIWA_ERR_LEVEL
+    assert call(storage(ok_axis, ok_axis), lmax=(4, 63)) == _lib.PALIWA_ERR_LEVEL
+    assert call(storage(ok_axis, ok_axis), lmin=(4, 0)) == _lib.PALIWA_ERR_LEVEL
+    assert call(storage(ok_axis, ok_axis), name=b"haar") == _lib.PALIWA_ERR_UNKNOWN_WAVELET
+    assert call(storage(ok_axis, ok_axis), dtype=5) == _lib.PALIWA_ERR_DTYPE
+    assert call(None) == _lib.PALIWA_ERR_INVALID_ARGUMENT
+    # an empty buffer is valid and does nothing
+    assert call(storage(ok_axis, (0, 0, 1, None))) == _lib.PALIWA_OK
+    fn = _lib.lib.paliwa_transform_in_checked
+    filt = _lib.filter_array([(2, (0.0, 1.0, 0.0))])
+    st = storage(ok_axis, ok_axis)
+    assert fn(FAKE, 0, 2, st, _lib.i32_array([3, 3]), _lib.i32_array([4, 4]), 0,
+              _lib.i32_array([3]), 1, filt, 1, None) == _lib.PALIWA_ERR_FILTER_OFFSET
+    good = _lib.filter_array([(1, (0.0, 1.0, 0.0))])
+    assert fn(FAKE, 0, 2, st, _lib.i32_array([3, 3]), _lib.i32_array([4, 4]), 0,
+              _lib.i32_array([4]), 1, good, 1, None) == _lib.PALIWA_ERR_LEVEL
+    assert fn(FAKE, 0, 2, st, _lib.i32_array([3, 3]), _lib.i32_array([4, 4]), 2,
+              _lib.i32_array([3]), 1, good, 1, None) == _lib.PALIWA_ERR_INVALID_ARGUMENT
