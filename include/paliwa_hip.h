@@ -464,6 +464,22 @@ int paliwa_learned_gradient(const float *pairs, int32_t stride,
                             const float *weights, float *grad,
                             float *loss_sum, float *workspace, void *stream);
 
+/* paliwa_learned_gradient without a workspace argument.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * per-workgroup partial sums live in a static device array of the library
+ * (1.4 MiB, one copy per GPU, held until the process ends), so count is at
+ * most PALIWA_LEARNED_STATIC_MAX_ROWS (else PALIWA_ERR_UNSUPPORTED).  Nothing
+ * is allocated; the first call on a device looks the array's address up
+ * (hipGetSymbolAddress) and keeps it.  Same kernels and result as
+ * paliwa_learned_gradient.  Calls that overlap on one GPU share the array and
+ * race: order them on one stream or with events. */
+#define PALIWA_LEARNED_STATIC_MAX_ROWS 16384
+int paliwa_learned_gradient_static(const float *pairs, int32_t stride,
+                                   const int32_t *index, int32_t count,
+                                   const float *weights, float *grad,
+                                   float *loss_sum, void *stream);
+
 /* One Adam step (step = 1, 2, ...) on the fp32 weights; clears grad. */
 int paliwa_learned_adam(float *weights, float *grad, float *m1, float *m2,
                         float lr, float beta1, float beta2, float eps,

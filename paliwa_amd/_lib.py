@@ -152,6 +152,7 @@ SIGNATURES = {
     "paliwa_learned_mlp_eval": (ctypes.c_int, [_vp, _vp, _vp, _i32, _vp]),
     "paliwa_learned_gradient_workspace": (ctypes.c_int64, [_i32]),
     "paliwa_learned_gradient": (ctypes.c_int, [_vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "paliwa_learned_gradient_static": (ctypes.c_int, [_vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp]),
     "paliwa_learned_adam": (
         ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                        ctypes.c_float, _i32, _vp]),

@@ -710,6 +710,36 @@ uint32_t blocks_for(int64_t work, int threads) {
 }
 
 } // namespace learned
+
+// Partial-sum floats of one gradient evaluation over `count` rows (one row of
+// kPartial per workgroup of kTrainThreads rows).
+int64_t learned_gradient_floats(int32_t count) {
+  using namespace learned;
+  return count < 1 ? 0 : int64_t(blocks_for(count, kTrainThreads)) * kPartial;
+}
+
+// Largest row count of a gradient evaluation (one workgroup per 64 rows, at
+// most 2^20 workgroups).
+int32_t learned_gradient_max_rows() { return (1 << 20) * learned::kTrainThreads; }
+
+// The two launches of a gradient evaluation: per-workgroup partial sums into
+// `partials` (learned_gradient_floats(count) floats, every one written), then
+// their sum in workgroup order into grad and *loss_sum.
+hipError_t launch_learned_gradient(float const *pairs, int32_t stride,
+                                   int32_t const *index, int32_t count,
+                                   float const *weights, float *grad,
+                                   float *loss_sum, float *partials,
+                                   hipStream_t stream) {
+  using namespace learned;
+  int32_t const blocks = static_cast<int32_t>(blocks_for(count, kTrainThreads));
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks), dim3(kTrainThreads), 0, stream,
+                     pairs, stride, index, count, weights, partials,
+                     1.0f / float(count));
+  hipLaunchKernelGGL(reduce_partials, dim3((kPartial + 255) / 256), dim3(256), 0,
+                     stream, partials, blocks, grad, loss_sum);
+  return hipGetLastError();
+}
+
 } // namespace paliwa_hip
 
 using namespace paliwa_hip;
@@ -915,7 +945,7 @@ int paliwa_learned_mlp_eval(float const *inputs, float const *weights,
 }
 
 int64_t paliwa_learned_gradient_workspace(int32_t count) {
-  return count < 1 ? 0 : int64_t(blocks_for(count, kTrainThreads)) * kPartial;
+  return learned_gradient_floats(count);
 }
 
 int paliwa_learned_gradient(float const *pairs, int32_t stride,
@@ -925,14 +955,11 @@ int paliwa_learned_gradient(float const *pairs, int32_t stride,
   if (pairs == nullptr || weights == nullptr || grad == nullptr ||
       loss_sum == nullptr || workspace == nullptr || count < 1 || stride < 9)
     return fail(PALIWA_ERR_INVALID_ARGUMENT, "bad arguments");
-  auto const s = static_cast<hipStream_t>(stream);
-  int32_t const blocks = static_cast<int32_t>(blocks_for(count, kTrainThreads));
-  hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks), dim3(kTrainThreads), 0, s,
-                     pairs, stride, index, count, weights, workspace,
-                     1.0f / float(count));
-  hipLaunchKernelGGL(reduce_partials, dim3((kPartial + 255) / 256), dim3(256), 0,
-                     s, workspace, blocks, grad, loss_sum);
-  return hip_status(hipGetLastError());
+  if (count > learned_gradient_max_rows())
+    return fail(PALIWA_ERR_UNSUPPORTED, "at most 2^26 rows per gradient");
+  return hip_status(launch_learned_gradient(pairs, stride, index, count, weights, grad,
+                                            loss_sum, workspace,
+                                            static_cast<hipStream_t>(stream)));
 }
 
 int paliwa_learned_adam(float *weights, float *grad, float *m1, float *m2,
