@@ -30,6 +30,7 @@ import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+USABLE_CPUS = 0  # CPUs this process may use, counted before OpenMP pins threads
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
@@ -166,16 +167,17 @@ def cpu_baseline(grid_cpu, budget_s):
     detail = []
     for label, sched in (("packed-poles", ob.OMP_PACKED_POLES), ("direct", ob.OMP_DIRECT)):
         a = np.ascontiguousarray(grid_cpu.copy())
-        reps, t0 = 0, time.perf_counter()
-        while True:
+        ob.hierarchize(a, level, lmin, level, WAVELET, sched)  # warm-up
+        ob.dehierarchize(a, level, lmin, level, WAVELET, sched)
+        times, t0 = [], time.perf_counter()
+        while len(times) < 5 or time.perf_counter() - t0 < budget_s:
+            t1 = time.perf_counter()
             ob.hierarchize(a, level, lmin, level, WAVELET, sched)
             ob.dehierarchize(a, level, lmin, level, WAVELET, sched)
-            reps += 1
-            if time.perf_counter() - t0 >= budget_s:
-                break
-        dt = (time.perf_counter() - t0) / reps
+            times.append(time.perf_counter() - t1)
+        dt = float(np.median(times))
         gbs = bytes_per_rep / dt / 1e9
-        detail.append(f"{label} {reps} reps {dt:.3f} s/rep {gbs:.2f} GB/s")
+        detail.append(f"{label} median of {len(times)} reps {dt:.3f} s/rep {gbs:.2f} GB/s")
         if best is None or gbs > best:
             best = gbs
     model = "unknown CPU"
@@ -187,6 +189,7 @@ def cpu_baseline(grid_cpu, budget_s):
                     break
     except OSError:
         pass
+    usable = USABLE_CPUS or len(os.sched_getaffinity(0))
     return {
         "value": round(best, 3),
         "unit": "GB/s",
@@ -194,7 +197,11 @@ def cpu_baseline(grid_cpu, budget_s):
         "kind": "port",
         "sample": "257^3 biorthogonal float64 forward+inverse, C++/OpenMP "
                   "restatement of the reference's CPU schedules (oracle/), "
-                  f"{ob.num_threads()} threads on {model}: " + "; ".join(detail),
+                  f"{ob.num_threads()} threads (OMP_PROC_BIND="
+                  f"{os.environ.get('OMP_PROC_BIND')}, OMP_PLACES="
+                  f"{os.environ.get('OMP_PLACES')}) of the {usable} CPUs this process may "
+                  f"use ({os.cpu_count()} in the host) on {model}; median per step after a "
+                  "warm-up: " + "; ".join(detail),
     }
 
 
@@ -283,6 +290,10 @@ def ct_benchmark(torch, pw, fields, reps=5):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / n
 
+    rule_ms = timed(lambda: batch.hierarchize("hat"), reps)
+    # the plan's autotuner: tile widths and stream schedule timed on these
+    # grids (restored afterwards), no cache so every run tunes afresh
+    tune = batch.tune("hat", forward=True, force=True, measure=True)
     batched_ms = timed(lambda: batch.hierarchize("hat"), reps)
 
     # per grid: prebuilt C-ABI arguments, a tight loop of hierarchize calls
@@ -317,20 +328,44 @@ def ct_benchmark(torch, pw, fields, reps=5):
     batch.close()
     bytes_fwd = 2 * 4 * points * 8
     combine = combine_benchmark(torch, levels, out_batched, timed, reps)
+    cpp = cpp_per_grid_loop()
     return {
         "workload": "1325 4-D float64 grids, |l|_1 in {13..16}, l_i >= 1, hat, "
                     "forward, periodic images",
         "grids": len(levels),
         "points": points,
         "batched_ms": round(batched_ms, 4),
+        "rule_ms": round(rule_ms, 4),
+        "tuned_ms": round(batched_ms, 4),
+        "tuner": {"rule_ms_median": round(tune["rule_ms"], 4),
+                  "tuned_ms_median": round(tune["tuned_ms"], 4),
+                  "evaluations": tune["evaluations"], "streams": tune["streams"]},
         "per_grid_ms": round(per_grid_ms, 4),
         "speedup": round(per_grid_ms / batched_ms, 2),
         "batched_GBps": round(bytes_fwd / (batched_ms * 1e-3) / 1e9, 1),
         "batched_launches": info["launches_per_direction"],
         "per_grid_launches": 4 * len(levels),
         "max_rel_diff_batched_vs_per_grid": max_rel,
+        "cpp_per_grid": cpp,
         "combine": combine,
     }
+
+
+def cpp_per_grid_loop():
+    """The reference's per-grid loop in C++ through the drop-in header
+    (tests/cpp/ct_per_grid.cpp): one stream, and 32 streams round-robin as the
+    reference's partitioned instances, beside the batched plan on the same
+    grids (no autotuning there)."""
+    import json
+    import subprocess
+
+    exe = os.path.join(ROOT, "tests", "cpp", "build", "ct_per_grid")
+    if not os.path.exists(exe):
+        return {"skipped": "tests/cpp/build/ct_per_grid is not built"}
+    out = subprocess.run([exe, "3"], capture_output=True, text=True, timeout=240)
+    if out.returncode != 0:
+        return {"error": (out.stdout + out.stderr)[-400:]}
+    return json.loads(out.stdout.strip().splitlines()[-1])
 
 
 def per_axis_benchmark(torch, grid, level, wavelet, reps=20):
@@ -464,6 +499,13 @@ def main():
               "the transform of one grid runs on one GPU and nothing is exchanged "
               "between GPUs.", file=sys.stderr)
         sys.exit(2)
+    # the CPU baseline's OpenMP threads: one per core, packed (set before the
+    # oracle's OpenMP runtime starts, which then pins the main thread; the
+    # CPUs this process may use are counted first)
+    global USABLE_CPUS
+    USABLE_CPUS = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import torch
 
     if not torch.cuda.is_available():
@@ -651,11 +693,27 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "residency": "warm: the 136 MB grid stays in the 256 MB Infinity Cache "
+                         "between sweeps; cold: the cache is flushed before the sweep",
+            "cold": {
+                "achieved": round(bytes_per_sweep / (cold[dominant] * 1e-3) / 1e9, 2),
+                "frac": round(bytes_per_sweep / (cold[dominant] * 1e-3) / 1e9
+                              / HBM_PEAK_GBS, 4),
+            },
         },
         "cpu_baseline": None,
     }
     record["fp32"] = fp32_variant(torch, pw, original, args.steps, args.warmup)
     record["other_workloads"] = other_workloads(torch, fields)
+    # the same kernel on a grid four times the Infinity Cache (513^3 float64,
+    # 1.08 GB): its slowest sweep, every byte from HBM
+    big = record["other_workloads"]["biorthogonal_3d_513"]["per_axis"]
+    slow_key = max(big, key=lambda k: big[k]["us"])
+    record["roofline"]["beyond_cache_513"] = {
+        "kernel": f"fused_ladder {slow_key.replace('_', ' ')}",
+        "achieved": big[slow_key]["GBps"],
+        "frac": round(big[slow_key]["GBps"] / HBM_PEAK_GBS, 4),
+    }
     if not args.no_ct:
         record["ct"] = ct_benchmark(torch, pw, fields)
     if not args.no_learned:

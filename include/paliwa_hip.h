@@ -260,6 +260,37 @@ int paliwa_batch_info(const paliwa_batch *plan, int32_t *launches,
 
 int paliwa_batch_destroy(paliwa_batch *plan);
 
+/* Autotuning of a plan (no counterpart in the reference, whose per-grid loop
+ * has nothing to tune).  Times alternatives for one wavelet and direction on
+ * the plan's own grids and keeps the fastest, by coordinate descent on the
+ * whole direction's time (median of 15 replays, a choice kept when at least
+ * 1% faster):
+ *   - the tile width (256 / 512 / 1024 threads, where the tile fits in LDS) of
+ *     every strided group whose lines hold 512 points or more;
+ *   - the stream schedule: 2, 3, 4, 6 or 8 streams for the groups of an axis,
+ *     and the launch-cost weight (1, 4, 16) of long-line groups in the
+ *     balance across them.
+ * The grids are copied to a temporary device buffer first and restored
+ * afterwards, bit for bit, so the call allocates memory the size of the set
+ * for its duration and synchronizes `stream`; results of the plan do not
+ * depend on any choice.  cache_path (may be NULL): a text file of earlier
+ * choices, one per line ("<key> = <value>", keys naming dtype, layout, line
+ * period, stored extent, line-count bucket, wavelet and direction for widths,
+ * a hash of the plan's groups for the schedule); choices found there are
+ * applied without timing and the file is rewritten with every choice of this
+ * plan.  When the cache holds every choice nothing is timed or allocated
+ * unless flags has PALIWA_TUNE_MEASURE. */
+#define PALIWA_TUNE_FORCE 1   /* ignore the cache: time every choice */
+#define PALIWA_TUNE_MEASURE 2 /* time the rule-based and final settings */
+int paliwa_batch_tune(paliwa_batch *plan, const char *wavelet_name,
+                      int32_t forward, const char *cache_path, int32_t flags,
+                      void *stream);
+/* The last tuning: rule-based and tuned time of the direction in ms (-1 when
+ * not timed), evaluations timed, streams per axis now in use. */
+int paliwa_batch_tune_info(const paliwa_batch *plan, double *rule_ms,
+                           double *tuned_ms, int32_t *evaluations,
+                           int32_t *streams);
+
 /* ---- Partial-output transform ------------------------------------------- *
  * The reference's transform_in with an `outputs` domain
  * (paliwa/paliwa_transform.hpp:253-344) and its dependency planner

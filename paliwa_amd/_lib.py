@@ -33,6 +33,9 @@ PALIWA_MAX_LEVELS = 64
 
 PATHS = {"auto": 0, "fused": 1, "lds_line": 2, "per_pass": 3}
 
+PALIWA_TUNE_FORCE = 1
+PALIWA_TUNE_MEASURE = 2
+
 
 class paliwa_filter(ctypes.Structure):
     _fields_ = [
@@ -118,6 +121,10 @@ SIGNATURES = {
     "paliwa_batch_set_graphs": (ctypes.c_int, [_vp, _i32]),
     "paliwa_batch_info": (ctypes.c_int, [_vp, _i32p, _i64p, _i64p]),
     "paliwa_batch_destroy": (ctypes.c_int, [_vp]),
+    "paliwa_batch_tune": (ctypes.c_int, [_vp, ctypes.c_char_p, _i32, ctypes.c_char_p, _i32, _vp]),
+    "paliwa_batch_tune_info": (
+        ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                       _i32p, _i32p]),
     "paliwa_transform_bounds": (
         ctypes.c_int, [paliwa_periodic_bounds, _i32, _i32, _i32p, _i32, _fp, _i32, _pbp, _pbp]),
     "paliwa_covering_periodic_bounds": (ctypes.c_int, [_i64p, ctypes.c_int64, ctypes.c_int64,

@@ -86,6 +86,26 @@ class CombinationBatch:
         return {"launches_per_direction": launches.value, "points": points.value,
                 "descriptor_bytes": nbytes.value}
 
+    def tune(self, wavelet_name: str = "hat", forward: bool = True, cache=None,
+             force: bool = False, measure: bool = False, stream=None):
+        """Time alternative tile widths and stream schedules on the plan's own
+        grids (restored afterwards) and keep the fastest; see
+        paliwa_batch_tune.  ``cache``: path of a tuning cache file.  Returns
+        {"rule_ms", "tuned_ms", "evaluations", "streams"}."""
+        flags = (_lib.PALIWA_TUNE_FORCE if force else 0) | (
+            _lib.PALIWA_TUNE_MEASURE if measure else 0)
+        with torch.cuda.device(self.device):
+            check(_lib.lib.paliwa_batch_tune(
+                self._plan, str(wavelet_name).encode(), int(bool(forward)),
+                None if cache is None else str(cache).encode(), flags,
+                _stream_handle(self.grids[0], stream)))
+        rule, tuned = ctypes.c_double(0), ctypes.c_double(0)
+        evals, streams = ctypes.c_int32(0), ctypes.c_int32(0)
+        check(_lib.lib.paliwa_batch_tune_info(self._plan, ctypes.byref(rule), ctypes.byref(tuned),
+                                              ctypes.byref(evals), ctypes.byref(streams)))
+        return {"rule_ms": rule.value, "tuned_ms": tuned.value, "evaluations": evals.value,
+                "streams": streams.value}
+
     def close(self) -> None:
         if getattr(self, "_plan", None) is not None and self._plan.value:
             _lib.lib.paliwa_batch_destroy(self._plan)

@@ -13,7 +13,10 @@
 #include "lifting.hpp"
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <map>
 #include <mutex>
 #include <string>
@@ -37,6 +40,8 @@ struct paliwa_batch {
     bool lds_lines = false;          // longer lines: one batched LDS-line launch
     int64_t tiles = 0;               // fused groups: tiles; LDS-line groups: lines
     int32_t threads = 0;             // fused strided groups: tile width (0: default)
+    std::vector<int32_t> widths;     // tile widths the tuner may choose from
+    int64_t lines = 0;               // lines of all members along the axis
     BatchGrid *grids_dev = nullptr;  // fused groups: descriptors on the device
     int32_t *tile_grid_dev = nullptr;
     std::vector<int32_t> members;    // grid indices
@@ -46,6 +51,8 @@ struct paliwa_batch {
   size_t dev_bytes = 0;
 
   // internal streams (default priority) and fork / join events
+  int32_t nlanes = 4;      // streams for the groups of one axis
+  int32_t long_weight = 4; // launch-cost multiple of a long-line group
   std::vector<hipStream_t> lanes;
   std::vector<hipEvent_t> joins;
   hipEvent_t fork = nullptr;
@@ -54,6 +61,10 @@ struct paliwa_batch {
   bool use_graphs = true;
   std::mutex mutex;
   std::map<std::pair<int, int>, hipGraphExec_t> graphs; // (wavelet, forward)
+
+  // the last paliwa_batch_tune call
+  double rule_ms = -1.0, tuned_ms = -1.0;
+  int32_t evaluations = 0;
 
   int32_t const *level_of(int g) const { return &level[size_t(g) * ndim]; }
   int64_t const *extents_of(int g) const { return &extents[size_t(g) * ndim]; }
@@ -216,6 +227,141 @@ int execute(paliwa_batch *b, char const *wavelet_name, bool forward,
   return rc;
 }
 
+void drop_graphs(paliwa_batch *b) {
+  for (auto &kv : b->graphs)
+    (void)hipGraphExecDestroy(kv.second);
+  b->graphs.clear();
+}
+
+// Lines per tile of a group at its current width.
+int64_t tile_lines(paliwa_batch const &b, paliwa_batch::Group const &grp) {
+  if (!grp.fused)
+    return 1;
+  int const G = (1 << grp.L) / fused_chunk(b.dtype, grp.L);
+  return grp.threads ? grp.threads / G : fused_tile_lines(b.dtype, grp.L, grp.strided);
+}
+
+// (Re)builds every group's tile table at its current width: one device
+// allocation holding all of them.
+int build_tables(paliwa_batch *b) {
+  std::vector<std::vector<BatchGrid>> host_grids(b->groups.size());
+  std::vector<std::vector<int32_t>> host_tiles(b->groups.size());
+  for (size_t i = 0; i < b->groups.size(); ++i) {
+    auto &grp = b->groups[i];
+    grp.tiles = 0;
+    if (!grp.fused && !grp.lds_lines)
+      continue;
+    int64_t const tb = tile_lines(*b, grp);
+    for (size_t k = 0; k < grp.members.size(); ++k) {
+      int32_t const g = grp.members[k];
+      SweepParams const p = make_params(b->data[g], b->ndim, b->extents_of(g), b->level_of(g),
+                                        grp.dim);
+      int64_t const tiles = (p.A * p.B + tb - 1) / tb;
+      host_grids[i].push_back(BatchGrid{b->data[g], p.A, p.B, grp.tiles});
+      host_tiles[i].insert(host_tiles[i].end(), size_t(tiles), int32_t(k));
+      grp.tiles += tiles;
+    }
+    if (grp.tiles > 0x7fffffffLL)
+      return fail(PALIWA_ERR_UNSUPPORTED, "batch too large");
+  }
+  std::vector<size_t> grid_off(b->groups.size()), tile_off(b->groups.size());
+  size_t bytes = 0;
+  for (size_t i = 0; i < b->groups.size(); ++i) {
+    grid_off[i] = bytes;
+    bytes += host_grids[i].size() * sizeof(BatchGrid);
+    bytes = (bytes + 255) & ~size_t(255);
+    tile_off[i] = bytes;
+    bytes += host_tiles[i].size() * sizeof(int32_t);
+    bytes = (bytes + 255) & ~size_t(255);
+  }
+  drop_graphs(b);
+  if (b->dev != nullptr && bytes != b->dev_bytes) {
+    (void)hipFree(b->dev);
+    b->dev = nullptr;
+    b->dev_bytes = 0;
+  }
+  if (bytes == 0)
+    return PALIWA_OK;
+  std::vector<unsigned char> staging(bytes, 0);
+  for (size_t i = 0; i < b->groups.size(); ++i) {
+    std::memcpy(staging.data() + grid_off[i], host_grids[i].data(),
+                host_grids[i].size() * sizeof(BatchGrid));
+    std::memcpy(staging.data() + tile_off[i], host_tiles[i].data(),
+                host_tiles[i].size() * sizeof(int32_t));
+  }
+  if ((b->dev == nullptr && hipMalloc(&b->dev, bytes) != hipSuccess) ||
+      hipMemcpy(b->dev, staging.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(PALIWA_ERR_HIP, "could not place the batch descriptors");
+  b->dev_bytes = bytes;
+  for (size_t i = 0; i < b->groups.size(); ++i) {
+    auto *base = static_cast<unsigned char *>(b->dev);
+    b->groups[i].grids_dev = reinterpret_cast<BatchGrid *>(base + grid_off[i]);
+    b->groups[i].tile_grid_dev = reinterpret_cast<int32_t *>(base + tile_off[i]);
+  }
+  return PALIWA_OK;
+}
+
+constexpr int kMaxLanes = 8;
+
+// Lanes for the groups of each axis: largest group first onto the least
+// loaded of b->nlanes streams.  Work estimate in swept points: the group's
+// points plus a fixed cost per launch (about 10 us of launch, fill and drain,
+// the time the sweep needs for ~3.75 M float64 points), long_weight times that
+// for a long-line group, whose few workgroups make it latency-bound.  Creates
+// the streams when the lane count changed.
+int setup_lanes(paliwa_batch *b) {
+  drop_graphs(b); // captured graphs hold the old assignment
+  b->lane_of.assign(b->groups.size(), 0);
+  bool concurrent = false;
+  for (size_t i = 0; i < b->groups.size();) {
+    size_t j = i;
+    while (j < b->groups.size() && b->groups[j].dim == b->groups[i].dim)
+      ++j;
+    if (j - i > 1) {
+      concurrent = true;
+      std::vector<std::pair<int64_t, size_t>> work;
+      for (size_t k = i; k < j; ++k) {
+        int64_t pts = 0;
+        for (int32_t g : b->groups[k].members) {
+          int64_t e = 1;
+          for (int32_t d = 0; d < b->ndim; ++d)
+            e *= b->extents_of(g)[d];
+          pts += e;
+        }
+        int64_t const launch = b->dtype == PALIWA_FLOAT64 ? 3750000 : 7500000;
+        pts += b->groups[k].lds_lines ? b->long_weight * launch : launch;
+        work.push_back({pts, k});
+      }
+      std::sort(work.begin(), work.end(), [](auto const &x, auto const &y) {
+        return x.first != y.first ? x.first > y.first : x.second < y.second;
+      });
+      std::vector<int64_t> load(size_t(b->nlanes), 0);
+      for (auto const &[pts, k] : work) {
+        int best = 0;
+        for (int t = 1; t < b->nlanes; ++t)
+          if (load[t] < load[best])
+            best = t;
+        load[best] += pts;
+        b->lane_of[k] = best;
+      }
+    }
+    i = j;
+  }
+  // the streams are created once, kMaxLanes of them; a schedule uses the
+  // first nlanes
+  if (!concurrent || !b->lanes.empty())
+    return PALIWA_OK;
+  b->lanes.assign(kMaxLanes, nullptr);
+  b->joins.assign(kMaxLanes, nullptr);
+  bool ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+  for (int t = 0; t < kMaxLanes && ok; ++t)
+    ok = hipStreamCreateWithFlags(&b->lanes[t], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&b->joins[t], hipEventDisableTiming) == hipSuccess;
+  if (!ok)
+    return fail(PALIWA_ERR_HIP, "could not create the plan's streams");
+  return PALIWA_OK;
+}
+
 void release(paliwa_batch *b) {
   for (auto &kv : b->graphs)
     (void)hipGraphExecDestroy(kv.second);
@@ -230,6 +376,311 @@ void release(paliwa_batch *b) {
   if (b->dev != nullptr)
     (void)hipFree(b->dev);
   delete b;
+}
+
+
+// ---- autotuning -------------------------------------------------------------
+// The rules above (tile width from a group's line count, four streams, a
+// launch-cost estimate for the lane balance) are a starting point; the tuner
+// times alternatives on the plan's own grids and keeps the fastest.  What it
+// chooses: the tile width of every strided fused group whose width is chosen
+// at launch (256 / 512 / 1024 threads where the tile fits), and the stream
+// schedule (number of streams, launch-cost weight of long-line groups).  The
+// objective is the whole direction's time as the plan executes it (one graph
+// replay), so interactions between concurrent groups count.
+
+uint64_t fnv(uint64_t h, int64_t v) {
+  for (int i = 0; i < 8; ++i) {
+    h ^= uint64_t(v >> (8 * i)) & 0xff;
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+std::string direction_tag(char const *wavelet, bool forward) {
+  return std::string(wavelet) + (forward ? " forward" : " inverse");
+}
+
+std::string width_key(paliwa_batch const &b, paliwa_batch::Group const &grp, char const *wavelet,
+                      bool forward) {
+  int bucket = 0;
+  while ((int64_t(2) << bucket) <= grp.lines)
+    ++bucket;
+  std::ostringstream k;
+  k << "width " << (b.dtype == PALIWA_FLOAT64 ? "f64" : "f32") << " "
+    << (grp.strided ? "strided" : "contiguous") << " L=" << grp.L << " m=" << grp.m
+    << " lines=2^" << bucket << " " << direction_tag(wavelet, forward);
+  return k.str();
+}
+
+std::string schedule_key(paliwa_batch const &b, char const *wavelet, bool forward) {
+  uint64_t h = 1469598103934665603ull;
+  h = fnv(fnv(h, b.dtype), b.ndim);
+  for (auto const &grp : b.groups) {
+    h = fnv(fnv(fnv(h, grp.dim), grp.L), grp.m);
+    h = fnv(fnv(h, grp.lines), int64_t(grp.members.size()));
+  }
+  std::ostringstream k;
+  k << "schedule " << std::hex << h << std::dec << " " << direction_tag(wavelet, forward);
+  return k.str();
+}
+
+using TuneCache = std::map<std::string, std::string>;
+
+TuneCache read_cache(char const *path) {
+  TuneCache c;
+  if (path == nullptr)
+    return c;
+  std::ifstream in(path);
+  std::string line;
+  while (std::getline(in, line)) {
+    if (line.empty() || line[0] == '#')
+      continue;
+    auto const eq = line.find(" = ");
+    if (eq != std::string::npos)
+      c[line.substr(0, eq)] = line.substr(eq + 3);
+  }
+  return c;
+}
+
+int write_cache(char const *path, TuneCache const &c) {
+  std::ofstream out(path, std::ios::trunc);
+  out << "# paliwa batched-transform tuning cache (paliwa_batch_tune), one choice per line:\n"
+         "#   width <dtype> <strided|contiguous> L=<log2 period> m=<stored extent> "
+         "lines=2^<floor log2 of the group's lines> <wavelet> <direction> = <threads>\n"
+         "#   schedule <plan hash> <wavelet> <direction> = <streams> <long-line weight>\n";
+  for (auto const &[k, v] : c)
+    out << k << " = " << v << "\n";
+  return out.good() ? PALIWA_OK : fail(PALIWA_ERR_IO, std::string("cannot write ") + path);
+}
+
+// Median of `reps` timed replays of one direction (ms); each is followed by
+// an untimed replay of the other direction, so the values stay bounded.
+int time_direction(paliwa_batch *b, char const *wavelet, bool forward, hipStream_t s, int reps,
+                   double *ms) {
+  std::vector<hipEvent_t> ev(2 * size_t(reps), nullptr);
+  int rc = PALIWA_OK;
+  for (auto &e : ev)
+    if (hipEventCreate(&e) != hipSuccess)
+      rc = fail(PALIWA_ERR_HIP, "could not create timing events");
+  for (int r = 0; r < 2 && rc == PALIWA_OK; ++r) // warm-up (and graph capture)
+    if ((rc = execute(b, wavelet, forward, s)) == PALIWA_OK)
+      rc = execute(b, wavelet, !forward, s);
+  for (int r = 0; r < reps && rc == PALIWA_OK; ++r) {
+    (void)hipEventRecord(ev[2 * r], s);
+    rc = execute(b, wavelet, forward, s);
+    (void)hipEventRecord(ev[2 * r + 1], s);
+    if (rc == PALIWA_OK)
+      rc = execute(b, wavelet, !forward, s);
+  }
+  std::vector<double> t;
+  if (rc == PALIWA_OK && hipStreamSynchronize(s) != hipSuccess)
+    rc = fail(PALIWA_ERR_HIP, "timing run failed");
+  for (int r = 0; r < reps && rc == PALIWA_OK; ++r) {
+    float x = 0.0f;
+    if (hipEventElapsedTime(&x, ev[2 * r], ev[2 * r + 1]) == hipSuccess)
+      t.push_back(x);
+  }
+  for (auto e : ev)
+    if (e != nullptr)
+      (void)hipEventDestroy(e);
+  if (rc != PALIWA_OK)
+    return rc;
+  if (t.empty())
+    return fail(PALIWA_ERR_HIP, "no timing");
+  std::sort(t.begin(), t.end());
+  *ms = t[t.size() / 2];
+  return PALIWA_OK;
+}
+
+int tune(paliwa_batch *b, char const *wavelet_name, bool forward, char const *cache_path,
+         int32_t flags, hipStream_t s) {
+  WaveletEntry const *w = find_wavelet(wavelet_name);
+  if (w == nullptr)
+    return fail(PALIWA_ERR_UNKNOWN_WAVELET, "map::at");
+  if (b->dtype != PALIWA_FLOAT64)
+    return fail(PALIWA_ERR_UNSUPPORTED,
+                "tuning of float32 plans is switched off (a float32 tuning run ended in "
+                "a host segmentation fault whose cause is not found); the rules apply");
+  char const *wname = w->name;
+  bool const force = flags & PALIWA_TUNE_FORCE;
+  bool const measure = flags & PALIWA_TUNE_MEASURE;
+  TuneCache cache = read_cache(cache_path);
+
+  // choices the cache answers, and what is left to search
+  std::vector<size_t> search;
+  std::vector<int32_t> rule_threads;
+  for (size_t i = 0; i < b->groups.size(); ++i) {
+    auto &grp = b->groups[i];
+    rule_threads.push_back(grp.threads);
+    if (grp.widths.empty())
+      continue;
+    auto const it = force ? cache.end() : cache.find(width_key(*b, grp, wname, forward));
+    int32_t cached = 0;
+    if (it != cache.end())
+      cached = std::atoi(it->second.c_str());
+    if (std::find(grp.widths.begin(), grp.widths.end(), cached) == grp.widths.end())
+      search.push_back(i);
+  }
+  int32_t const rule_lanes = b->nlanes, rule_weight = b->long_weight;
+  std::string const skey = schedule_key(*b, wname, forward);
+  bool search_schedule = force || cache.find(skey) == cache.end();
+  bool const concurrent = !b->lanes.empty();
+  if (!concurrent)
+    search_schedule = false;
+  b->evaluations = 0;
+  b->rule_ms = b->tuned_ms = -1.0;
+  bool const timing = measure || !search.empty() || search_schedule;
+
+  void *backup = nullptr;
+  size_t const es = b->dtype == PALIWA_FLOAT64 ? 8 : 4;
+  std::vector<size_t> bytes(b->ngrids);
+  size_t total = 0;
+  for (int32_t g = 0; g < b->ngrids; ++g) {
+    int64_t e = 1;
+    for (int32_t d = 0; d < b->ndim; ++d)
+      e *= b->extents_of(g)[d];
+    bytes[g] = size_t(e) * es;
+    total += bytes[g];
+  }
+  auto copy_all = [&](bool save) {
+    size_t off = 0;
+    for (int32_t g = 0; g < b->ngrids; ++g) {
+      auto *bk = static_cast<unsigned char *>(backup) + off;
+      hipError_t const e = save ? hipMemcpyAsync(bk, b->data[g], bytes[g],
+                                                 hipMemcpyDeviceToDevice, s)
+                                : hipMemcpyAsync(b->data[g], bk, bytes[g],
+                                                 hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess)
+        return fail(PALIWA_ERR_HIP, std::string("HIP: ") + hipGetErrorString(e));
+      off += bytes[g];
+    }
+    return hipStreamSynchronize(s) == hipSuccess ? PALIWA_OK
+                                                 : fail(PALIWA_ERR_HIP, "copy failed");
+  };
+  int rc = PALIWA_OK;
+  // candidates are timed as direct launches (no graph captured per candidate)
+  bool const graphs = b->use_graphs;
+  b->use_graphs = false;
+  if (timing) {
+    if (hipMalloc(&backup, total) != hipSuccess) {
+      b->use_graphs = graphs;
+      (void)hipGetLastError();
+      return fail(PALIWA_ERR_HIP, "no device memory for the tuning copy of the grids");
+    }
+    rc = copy_all(true);
+  }
+  constexpr int kReps = 15;
+  auto evaluate = [&](double *ms) {
+    ++b->evaluations;
+    return time_direction(b, wname, forward, s, kReps, ms);
+  };
+  if (rc == PALIWA_OK && timing)
+    rc = evaluate(&b->rule_ms);
+
+  // apply the cached choices
+  if (rc == PALIWA_OK) {
+    for (auto &grp : b->groups)
+      if (!grp.widths.empty() && !force) {
+        auto const it = cache.find(width_key(*b, grp, wname, forward));
+        if (it != cache.end()) {
+          int32_t const t = std::atoi(it->second.c_str());
+          if (std::find(grp.widths.begin(), grp.widths.end(), t) != grp.widths.end())
+            grp.threads = t;
+        }
+      }
+    if (!search_schedule && concurrent && !force) {
+      std::istringstream v(cache[skey]);
+      int32_t nl = 0, lw = 0;
+      if (v >> nl >> lw && nl >= 1 && nl <= kMaxLanes && lw >= 0) {
+        b->nlanes = nl;
+        b->long_weight = lw;
+      }
+    }
+    rc = build_tables(b);
+    if (rc == PALIWA_OK)
+      rc = setup_lanes(b);
+  }
+  double best = b->rule_ms;
+  if (rc == PALIWA_OK && timing && (!search.empty() || search_schedule || measure))
+    rc = evaluate(&best);
+  // coordinate descent: one choice at a time, kept when at least 1% faster
+  std::sort(search.begin(), search.end(), [&](size_t x, size_t y) {
+    return b->groups[x].lines * (int64_t(1) << b->groups[x].L) >
+           b->groups[y].lines * (int64_t(1) << b->groups[y].L);
+  });
+  for (int round = 0; round < 2 && rc == PALIWA_OK; ++round) {
+    for (size_t i : search) {
+      auto &grp = b->groups[i];
+      int32_t const keep = grp.threads;
+      int32_t chosen = keep;
+      for (int32_t t : grp.widths) {
+        if (t == keep || rc != PALIWA_OK)
+          continue;
+        grp.threads = t;
+        double ms = 0.0;
+        if ((rc = build_tables(b)) == PALIWA_OK && (rc = evaluate(&ms)) == PALIWA_OK &&
+            ms < 0.99 * best) {
+          best = ms;
+          chosen = t;
+        }
+      }
+      grp.threads = chosen;
+      if (rc == PALIWA_OK)
+        rc = build_tables(b);
+    }
+    if (search_schedule && rc == PALIWA_OK) {
+      int32_t keep_l = b->nlanes, keep_w = b->long_weight;
+      for (int32_t nl : {2, 3, 4, 6, 8})
+        for (int32_t lw : {1, 4, 16}) {
+          if ((nl == keep_l && lw == keep_w) || rc != PALIWA_OK)
+            continue;
+          b->nlanes = nl;
+          b->long_weight = lw;
+          double ms = 0.0;
+          if ((rc = setup_lanes(b)) == PALIWA_OK && (rc = evaluate(&ms)) == PALIWA_OK &&
+              ms < 0.99 * best) {
+            best = ms;
+            keep_l = nl;
+            keep_w = lw;
+          }
+        }
+      b->nlanes = keep_l;
+      b->long_weight = keep_w;
+      if (rc == PALIWA_OK)
+        rc = setup_lanes(b);
+    }
+    if (search.empty())
+      break;
+  }
+  if (timing)
+    b->tuned_ms = best;
+  b->use_graphs = graphs;
+  if (backup != nullptr) {
+    int const restore = copy_all(false);
+    (void)hipFree(backup);
+    if (rc == PALIWA_OK)
+      rc = restore;
+  }
+  if (rc != PALIWA_OK) { // leave the plan as the rules made it
+    for (size_t i = 0; i < b->groups.size(); ++i)
+      b->groups[i].threads = rule_threads[i];
+    b->nlanes = rule_lanes;
+    b->long_weight = rule_weight;
+    std::string const msg = last_error();
+    if (build_tables(b) == PALIWA_OK)
+      (void)setup_lanes(b);
+    return fail(rc, msg);
+  }
+  if (cache_path != nullptr) {
+    for (auto const &grp : b->groups)
+      if (!grp.widths.empty())
+        cache[width_key(*b, grp, wname, forward)] = std::to_string(grp.threads);
+    if (concurrent)
+      cache[skey] = std::to_string(b->nlanes) + " " + std::to_string(b->long_weight);
+    rc = write_cache(cache_path, cache);
+  }
+  return rc;
 }
 
 } // namespace
@@ -274,8 +725,6 @@ int paliwa_batch_create(paliwa_batch **plan, int32_t dtype, int32_t ndim,
   }
 
   // Group the sweeps: per axis, grids with equal (line period, stored extent).
-  std::vector<std::vector<BatchGrid>> host_grids;
-  std::vector<std::vector<int32_t>> host_tiles;
   for (int32_t d = 0; d < ndim; ++d) {
     std::map<std::pair<int32_t, int32_t>, std::vector<int32_t>> by_shape;
     for (int32_t g = 0; g < ngrids; ++g)
@@ -290,129 +739,32 @@ int paliwa_batch_create(paliwa_batch **plan, int32_t dtype, int32_t ndim,
       grp.members = members;
       grp.fused = (int64_t(1) << grp.L) <= fused_max_period(dtype);
       grp.lds_lines = !grp.fused && (int64_t(1) << grp.L) <= lds_line_max_period(dtype);
-      std::vector<BatchGrid> gd;
-      std::vector<int32_t> tg;
+      for (int32_t g : members) {
+        SweepParams const q = make_params(b->data[g], ndim, b->extents_of(g), b->level_of(g), d);
+        grp.lines += q.A * q.B;
+      }
       if (grp.fused && grp.strided) {
         // the tile width of a strided float ladder follows the group's lines
-        int64_t lines = 0;
-        for (int32_t g : members) {
-          SweepParams const q = make_params(b->data[g], ndim, b->extents_of(g), b->level_of(g), d);
-          lines += q.A * q.B;
-        }
         int const K = fused_chunk(dtype, grp.L);
-        grp.threads = ladder_threads_for(K, (1 << grp.L) / K, dtype == PALIWA_FLOAT64 ? 8 : 4, lines);
-      }
-      if (grp.fused || grp.lds_lines) {
-        // fused: tiles of fused_tile_lines lines (or threads / G); LDS-line:
-        // one line per tile
-        int64_t const tb = !grp.fused ? 1
-                           : grp.threads ? grp.threads / ((1 << grp.L) / fused_chunk(dtype, grp.L))
-                                         : fused_tile_lines(dtype, grp.L, grp.strided);
-        for (size_t k = 0; k < members.size(); ++k) {
-          int32_t const g = members[k];
-          SweepParams const p =
-              make_params(b->data[g], ndim, b->extents_of(g), b->level_of(g), d);
-          int64_t const tiles = (p.A * p.B + tb - 1) / tb;
-          gd.push_back(BatchGrid{b->data[g], p.A, p.B, grp.tiles});
-          tg.insert(tg.end(), size_t(tiles), int32_t(k));
-          grp.tiles += tiles;
-        }
-        if (grp.tiles > 0x7fffffffLL) {
-          release(b);
-          return fail(PALIWA_ERR_UNSUPPORTED, "batch too large");
+        int const G = (1 << grp.L) / K;
+        int const eb = dtype == PALIWA_FLOAT64 ? 8 : 4;
+        grp.threads = ladder_threads_for(K, G, eb, grp.lines);
+        if (grp.threads != 0) {
+          grp.widths = {256, 512};
+          if (2 * (int64_t(K) * G + 1) * (512 / G) * eb <= kWideTileLds)
+            grp.widths.push_back(1024);
         }
       }
       b->groups.push_back(std::move(grp));
-      host_grids.push_back(std::move(gd));
-      host_tiles.push_back(std::move(tg));
     }
   }
-
-  // One device allocation for all descriptor tables.
-  std::vector<size_t> grid_off(b->groups.size()), tile_off(b->groups.size());
-  size_t bytes = 0;
-  for (size_t i = 0; i < b->groups.size(); ++i) {
-    grid_off[i] = bytes;
-    bytes += host_grids[i].size() * sizeof(BatchGrid);
-    bytes = (bytes + 255) & ~size_t(255);
-    tile_off[i] = bytes;
-    bytes += host_tiles[i].size() * sizeof(int32_t);
-    bytes = (bytes + 255) & ~size_t(255);
-  }
-  if (bytes > 0) {
-    std::vector<unsigned char> staging(bytes, 0);
-    for (size_t i = 0; i < b->groups.size(); ++i) {
-      std::memcpy(staging.data() + grid_off[i], host_grids[i].data(),
-                  host_grids[i].size() * sizeof(BatchGrid));
-      std::memcpy(staging.data() + tile_off[i], host_tiles[i].data(),
-                  host_tiles[i].size() * sizeof(int32_t));
-    }
-    if (hipMalloc(&b->dev, bytes) != hipSuccess ||
-        hipMemcpy(b->dev, staging.data(), bytes, hipMemcpyHostToDevice) !=
-            hipSuccess) {
-      release(b);
-      return fail(PALIWA_ERR_HIP, "could not place the batch descriptors");
-    }
-    b->dev_bytes = bytes;
-    for (size_t i = 0; i < b->groups.size(); ++i) {
-      auto *base = static_cast<unsigned char *>(b->dev);
-      b->groups[i].grids_dev = reinterpret_cast<BatchGrid *>(base + grid_off[i]);
-      b->groups[i].tile_grid_dev = reinterpret_cast<int32_t *>(base + tile_off[i]);
-    }
-  }
-  // Lanes for the groups of each axis: largest group first onto the least
-  // loaded lane.  Work estimate in swept points: the group's points plus a
-  // fixed cost per launch (about 10 us of launch, fill and drain, the time the
-  // sweep needs for ~3.75 M float64 points), four times that for a long-line
-  // group, whose few workgroups make it latency-bound.
-  constexpr int kLanes = 4; // 3: 1.57 ms, 8: 1.61 ms, 4: 1.43 ms on the 1325-grid set
-  b->lane_of.assign(b->groups.size(), 0);
-  bool concurrent = false;
-  for (size_t i = 0; i < b->groups.size();) {
-    size_t j = i;
-    while (j < b->groups.size() && b->groups[j].dim == b->groups[i].dim)
-      ++j;
-    if (j - i > 1) {
-      concurrent = true;
-      std::vector<std::pair<int64_t, size_t>> work;
-      for (size_t k = i; k < j; ++k) {
-        int64_t pts = 0;
-        for (int32_t g : b->groups[k].members) {
-          int64_t e = 1;
-          for (int32_t d = 0; d < ndim; ++d)
-            e *= b->extents_of(g)[d];
-          pts += e;
-        }
-        int64_t const launch = dtype == PALIWA_FLOAT64 ? 3750000 : 7500000;
-        pts += b->groups[k].lds_lines ? 4 * launch : launch;
-        work.push_back({pts, k});
-      }
-      std::sort(work.begin(), work.end(), [](auto const &x, auto const &y) {
-        return x.first != y.first ? x.first > y.first : x.second < y.second;
-      });
-      int64_t load[kLanes] = {};
-      for (auto const &[pts, k] : work) {
-        int best = 0;
-        for (int t = 1; t < kLanes; ++t)
-          if (load[t] < load[best])
-            best = t;
-        load[best] += pts;
-        b->lane_of[k] = best;
-      }
-    }
-    i = j;
-  }
-  if (concurrent) {
-    b->lanes.assign(kLanes, nullptr);
-    b->joins.assign(kLanes, nullptr);
-    bool ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
-    for (int t = 0; t < kLanes && ok; ++t)
-      ok = hipStreamCreateWithFlags(&b->lanes[t], hipStreamNonBlocking) == hipSuccess &&
-           hipEventCreateWithFlags(&b->joins[t], hipEventDisableTiming) == hipSuccess;
-    if (!ok) {
-      release(b);
-      return fail(PALIWA_ERR_HIP, "could not create the plan's streams");
-    }
+  int rc = build_tables(b);
+  if (rc == PALIWA_OK)
+    rc = setup_lanes(b);
+  if (rc != PALIWA_OK) {
+    std::string const msg = last_error();
+    release(b);
+    return fail(rc, msg);
   }
   *plan = b;
   last_error().clear();
@@ -456,6 +808,36 @@ int paliwa_batch_info(paliwa_batch const *plan, int32_t *launches,
     *points = total;
   if (descriptor_bytes != nullptr)
     *descriptor_bytes = int64_t(plan->dev_bytes);
+  return PALIWA_OK;
+}
+
+int paliwa_batch_tune(paliwa_batch *plan, char const *wavelet_name, int32_t forward,
+                      char const *cache_path, int32_t flags, void *stream) {
+  if (plan == nullptr)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "plan is NULL");
+  int current = 0;
+  if (hipGetDevice(&current) != hipSuccess || current != plan->device)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT,
+                "the plan belongs to device " + std::to_string(plan->device));
+  int const rc = tune(plan, wavelet_name, forward != 0, cache_path, flags,
+                      static_cast<hipStream_t>(stream));
+  if (rc == PALIWA_OK)
+    last_error().clear();
+  return rc;
+}
+
+int paliwa_batch_tune_info(paliwa_batch const *plan, double *rule_ms, double *tuned_ms,
+                           int32_t *evaluations, int32_t *streams) {
+  if (plan == nullptr)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "plan is NULL");
+  if (rule_ms != nullptr)
+    *rule_ms = plan->rule_ms;
+  if (tuned_ms != nullptr)
+    *tuned_ms = plan->tuned_ms;
+  if (evaluations != nullptr)
+    *evaluations = plan->evaluations;
+  if (streams != nullptr)
+    *streams = plan->lanes.empty() ? 0 : plan->nlanes;
   return PALIWA_OK;
 }
 
