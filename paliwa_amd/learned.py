@@ -257,15 +257,19 @@ def detail_stats(coeffs: torch.Tensor, level, lmin, threshold: float):
 
 
 def workload(level=(9, 9, 9), train_level=(7, 7, 7), train_seeds=range(100, 132),
-             test_seed=4, pairs_count=1 << 22, steps=3000, batch=4096, lr=2e-3,
-             update=True, features=FEATURES_DETRENDED, device="cuda", log=None):
+             test_seed=4, validation_seed=200, pairs_count=1 << 22, steps=3000,
+             batch=4096, lr=2e-3, update=True, features=FEATURES_DETRENDED, device="cuda",
+             log=None):
     """The learned-lifting workload: train on turbulent-like fields (64
-    Fourier modes with a k^-5/3 spectrum plus 5 tanh fronts of width 0.01)
-    and compare, on a test field, the learned predict against the interpolet
-    (the same integer lifting with the fallback everywhere): fraction of detail
-    coefficients above 1e-4 max|f|, detail L1, and the forward+inverse round
-    trip.  Also reports the float interpolet of the plain library on the same
-    field."""
+    Fourier modes with a k^-5/3 spectrum plus 5 tanh fronts of width 0.01),
+    choose the per-level masks on a held-out validation field of the test
+    size (a seed outside the training and test seeds), and compare, on the
+    test field, the learned predict against the interpolet (the same integer
+    lifting with the fallback everywhere): fraction of detail coefficients
+    above 1e-4 max|f|, detail L1, and the forward+inverse round trip.  Also
+    reports masks probed on the test field itself (selection on the test
+    data, labelled as such) and the float interpolet of the plain library on
+    the same field."""
     import time
 
     from . import fields
@@ -301,14 +305,21 @@ def workload(level=(9, 9, 9), train_level=(7, 7, 7), train_seeds=range(100, 132)
     t_train = time.time() - t0
     say(f"train: loss {loss0.item():.4g} -> {loss1.item():.4g} in {t_train:.1f} s")
 
+    # masks on a held-out field of the test size
+    fv = fields.fourier_modes(level, modes=64, kmax=32, seed=validation_seed, device=device,
+                              fronts=5, front_width=0.01).to(torch.float32)
+    qv, _ = quantize(fv)
+    del fv
+    masks, trail = choose_masks(qv, level, lmin, weights, update, features, log=log)
+    del qv
+    say(f"masks (validation seed {validation_seed}): {[bin(m) for m in masks]}")
+
     # test field
     f = fields.fourier_modes(level, modes=64, kmax=32, seed=test_seed, device=device,
                              fronts=5, front_width=0.01).to(torch.float32)
     fmax = f.abs().max().item()
     thr = 1e-4 * fmax
     q, scale = quantize(f)
-    masks, trail = choose_masks(q, level, lmin, weights, update, features, log=log)
-    say(f"masks: {[bin(m) for m in masks]}")
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -350,6 +361,15 @@ def workload(level=(9, 9, 9), train_level=(7, 7, 7), train_seeds=range(100, 132)
     out["learned_without_fallback"] = {"frac_above_1e-4": frac, "detail_l1": l1,
                                        "forward_ms": round(ms_all, 3), "network_evals": evals}
     del c
+    # masks probed on the test field itself: selection on the test data, an
+    # upper bound on what the masks can give, reported apart
+    masks_test, _ = choose_masks(q, level, lmin, weights, update, features)
+    c = q.clone()
+    forward(c, level, lmin, weights, masks_test, update, features=features)
+    frac, l1 = detail_stats(c.double() * scale, level, lmin, thr)
+    out["learned_masks_from_test_field"] = {"frac_above_1e-4": frac, "detail_l1": l1,
+                                            "masks": masks_test}
+    del c
     # the plain float transform of the library (reference arithmetic), hat
     g = f.clone()
     plain.hierarchize(g, level, lmin, level, "hat")
@@ -359,6 +379,7 @@ def workload(level=(9, 9, 9), train_level=(7, 7, 7), train_seeds=range(100, 132)
                         "round_trip_rel": (g - f).abs().max().item() / fmax}
     out["overflow"] = int(overflow.item())
     out["masks"] = masks
+    out["mask_field_seed"] = validation_seed
     out["features"] = "detrended" if features == FEATURES_DETRENDED else "raw"
     out["update"] = bool(update)
     out["train"] = {"pairs": int(pairs.shape[0]), "steps": steps, "batch": batch,

@@ -216,11 +216,13 @@ def test_extract_pairs_and_train(ll):
 
 def test_learned_workload_gains_over_interpolet(ll):
     """The 513^3 float32 workload (training on 32 fields of 129^3, seeds
-    100-131; test field seed 4): the trained predictor with the interpolet
-    fallback lowers both the detail L1 and the fraction of details above
-    1e-4 max|f| against the interpolet alone, with the plain update kept and
-    an exact integer round trip."""
+    100-131; masks chosen on a held-out 513^3 field, seed 200; test field
+    seed 4): the trained predictor with the interpolet fallback lowers both
+    the detail L1 and the fraction of details above 1e-4 max|f| against the
+    interpolet alone on the test field, with the plain update kept and an
+    exact integer round trip (measured: 0.928 and 0.847)."""
     r = ll.workload()
+    assert r["mask_field_seed"] not in (4, *range(100, 132))
     li, le = r["interpolet"], r["learned"]
     assert r["overflow"] == 0
     assert le["integer_round_trip_exact"] and li["integer_round_trip_exact"]
