@@ -77,22 +77,28 @@ static void host_only_checks() {
   // Never dereferenced: every call below fails argument validation.
   paliwa::GridSpan<double, 2> g{reinterpret_cast<double *>(0x1000), {17, 16}};
   paliwa::LevelVector<2> lv{4, 4}, mn{0, 0}, mx{4, 4};
-  CHECK(throws<std::out_of_range>([&] { paliwa::hierarchize(g, lv, mn, mx, "haar"); }));
-  CHECK(throws<std::out_of_range>([&] { paliwa::dehierarchize_in<1>(g, lv, mn, mx, "db2"); }));
+  CHECK(throws<std::out_of_range>([&] { paliwa::hierarchize(g, lv, mn, mx, "haar", paliwa::HipExec{}); }));
+  CHECK(throws<std::out_of_range>([&] { paliwa::dehierarchize_in<1>(g, lv, mn, mx, "db2", paliwa::HipExec{}); }));
   std::vector<std::pair<int, std::array<double, 3>>> bad{{2, {0.0, 1.0, 0.0}}};
   std::vector<long> levels{4, 3};
   bool got = false;
   try {
-    paliwa::transform_in<0>(g, lv, mx, levels, bad);
+    paliwa::transform_in<0>(g, lv, mx, levels, bad, paliwa::HipExec{});
   } catch (std::runtime_error const &e) {
     got = std::strcmp(e.what(), "Filter offset not supported") == 0;
   }
   CHECK(got);
   paliwa::LevelVector<2> big{5, 0};
-  CHECK(throws<std::invalid_argument>([&] { paliwa::hierarchize(g, lv, big, mx, "hat"); }));
+  CHECK(throws<std::invalid_argument>([&] { paliwa::hierarchize(g, lv, big, mx, "hat", paliwa::HipExec{}); }));
   paliwa::GridSpan<double, 2> wrong{reinterpret_cast<double *>(0x1000), {18, 16}};
-  CHECK(throws<std::invalid_argument>([&] { paliwa::hierarchize(wrong, lv, mn, mx); }));
+  CHECK(throws<std::invalid_argument>([&] { paliwa::hierarchize(wrong, lv, mn, mx, "hat", paliwa::HipExec{}); }));
   CHECK(paliwa::lifting_wavelet_filter_offsets_and_coefficients.at("biorthogonal").size() == 2);
+  // a partial buffer whose coordinates leave the period is refused on the host
+  paliwa::PartialSpan<double, 2> part{reinterpret_cast<double *>(0x1000),
+                                      {paliwa_axis_storage{8, 10, 1, nullptr},
+                                       paliwa_axis_storage{4, 0, 1, nullptr}}};
+  CHECK(throws<std::invalid_argument>(
+      [&] { paliwa::hierarchize(part, lv, mn, mx, "hat", paliwa::HipExec{}); }));
   std::printf("host-only checks OK\n");
 }
 
@@ -171,7 +177,9 @@ static void grid_checks(hipStream_t stream) {
     std::vector<std::pair<int, std::array<double, 3>>> bio{{1, {-0.5, 1.0, -0.5}},
                                                            {0, {0.25, 1.0, 0.25}}};
     std::vector<long> lv_list{6, 5, 4, 3, 2, 1};
-    for (auto [start, count] : std::vector<std::pair<long, long>>{{120, 8}, {34, 5}, {0, 64}}) {
+    // {120, 8} twice: the second call reuses the plan the first one made
+    for (auto [start, count] :
+         std::vector<std::pair<long, long>>{{120, 8}, {34, 5}, {0, 64}, {120, 8}}) {
       HIP_CHECK(hipMemcpyAsync(d2, h2.data(), n2 * sizeof(T), hipMemcpyHostToDevice, stream));
       paliwa::transform_in<0>(g2, lv2, mx2, lv_list, bio, paliwa::AxisRun{start, count},
                               paliwa::HipExec{stream});
@@ -192,6 +200,41 @@ static void grid_checks(hipStream_t stream) {
                               paliwa::AxisRun{0, 4}, paliwa::HipExec{stream});
     }));
     HIP_CHECK(hipFree(d2));
+  }
+  // a partial buffer through the same names: a sub-box of a grid stored at
+  // its maximum level along axis 0 (a DDC DiscreteDomain block) and a strided
+  // run along axis 1, against the oracle's checked branch
+  {
+    constexpr std::size_t D3 = 2;
+    paliwa::LevelVector<D3> lv3{6, 4}, mn3{1, 0}, mx3{6, 5};
+    std::vector<std::vector<int64_t>> coords(2);
+    for (int64_t x = 5; x < 5 + 40; ++x)
+      coords[0].push_back(x);
+    for (int64_t y = 2; y < 2 + 2 * 12; y += 2)
+      coords[1].push_back(y);
+    std::size_t const n3 = coords[0].size() * coords[1].size();
+    std::vector<T> h3(n3);
+    for (auto &x : h3)
+      x = static_cast<T>(u(rng));
+    T *d3 = nullptr;
+    HIP_CHECK(hipMalloc(&d3, n3 * sizeof(T)));
+    paliwa::PartialSpan<T, D3> part{
+        d3, {paliwa_axis_storage{int64_t(coords[0].size()), 5, 1, nullptr},
+             paliwa_axis_storage{int64_t(coords[1].size()), 2, 2, nullptr}}};
+    for (char const *name : {"hat", "biorthogonal", "fullweighting"}) {
+      HIP_CHECK(hipMemcpyAsync(d3, h3.data(), n3 * sizeof(T), hipMemcpyHostToDevice, stream));
+      paliwa::hierarchize(part, lv3, mn3, mx3, name, paliwa::HipExec{stream});
+      std::vector<T> got3(n3), want3 = h3;
+      HIP_CHECK(hipMemcpyAsync(got3.data(), d3, n3 * sizeof(T), hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+      auto const &table = paliwa_oracle::forward_filters().at(name);
+      paliwa_oracle::transform_in_checked(want3.data(), coords, {6, 4}, {6, 5}, 0,
+                                          paliwa_oracle::decreasing_levels(6, 1), table);
+      paliwa_oracle::transform_in_checked(want3.data(), coords, {6, 4}, {6, 5}, 1,
+                                          paliwa_oracle::decreasing_levels(4, 0), table);
+      CHECK(std::memcmp(got3.data(), want3.data(), n3 * sizeof(T)) == 0);
+    }
+    HIP_CHECK(hipFree(d3));
   }
   // raw dump (paliwa/paliwa_io.hpp): the elements, row-major, headerless
   char const *tmp = std::getenv("TMPDIR");
