@@ -55,6 +55,9 @@ def parse():
                     help="skip the combination-technique batch measurement")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-lifting workload")
+    ap.add_argument("--ct-only", action="store_true",
+                    help="after the timed steps, only the combination-technique section "
+                         "(counter runs of the batched plan and the combine)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     return ap.parse_args()
@@ -720,6 +723,10 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if args.ct_only:
+        record["ct"] = ct_benchmark(torch, pw, fields)
+        print(json.dumps(record))
+        return
     record["fp32"] = fp32_variant(torch, pw, original, args.steps, args.warmup)
     record["other_workloads"] = other_workloads(torch, fields)
     # the same kernel on a grid four times the Infinity Cache (513^3 float64,
