@@ -422,6 +422,10 @@ def other_workloads(torch, fields):
         out[f"{wavelet}_6d_33x33x17x17x9x9"] = per_axis_benchmark(
             torch, g, [5, 5, 4, 4, 3, 3], wavelet)
     del g
+    # long lines (4097 points per axis): the long-line kernel on both axes
+    g = fields.gaussian_bump_2d((12, 12), seed=0, device="cuda")
+    out["biorthogonal_2d_4097"] = per_axis_benchmark(torch, g, [12, 12], "biorthogonal")
+    del g
     # a grid four times the Infinity Cache (1.08 GB): every sweep streams HBM
     g = fields.fourier_modes((9, 9, 9), modes=16, kmax=32, seed=1, device="cuda")
     out["biorthogonal_3d_513"] = per_axis_benchmark(torch, g, [9, 9, 9], "biorthogonal")

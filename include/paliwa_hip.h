@@ -70,6 +70,12 @@ extern "C" {
 #define PALIWA_PATH_PER_PASS 3 /* one launch per (level, pass) straight on HBM:
                                   the launch structure of the reference's
                                   detail::lifting_pass */
+#define PALIWA_PATH_LONG_LADDER 4 /* lines longer than the fused kernel holds
+                                     (float64 2^12..2^15, float32 2^13..2^16
+                                     points), named transforms' level lists:
+                                     one launch per axis, a tile of lines
+                                     across a workgroup, fine levels in
+                                     registers, coarse ones in LDS */
 
 /* One lifting pass: points of parity `offset` (0 even, 1 odd) at the operating
  * level get  g = c0*g[lower] + c1*g + c2*g[upper]  (evaluated left to right).

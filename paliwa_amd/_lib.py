@@ -31,7 +31,7 @@ PALIWA_MAX_DIMS = 26
 PALIWA_MAX_FILTERS = 16
 PALIWA_MAX_LEVELS = 64
 
-PATHS = {"auto": 0, "fused": 1, "lds_line": 2, "per_pass": 3}
+PATHS = {"auto": 0, "fused": 1, "lds_line": 2, "per_pass": 3, "long_ladder": 4}
 
 PALIWA_TUNE_FORCE = 1
 PALIWA_TUNE_MEASURE = 2

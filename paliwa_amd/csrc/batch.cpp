@@ -37,7 +37,8 @@ struct paliwa_batch {
     int32_t dim = 0, L = 0, m = 0;
     bool strided = false;
     bool fused = false;
-    bool lds_lines = false;          // longer lines: one batched LDS-line launch
+    bool long_ladder = false;        // longer lines: one batched long-line launch
+    bool lds_lines = false;          // lines beyond that: one batched LDS-line launch
     int64_t tiles = 0;               // fused groups: tiles; LDS-line groups: lines
     int32_t threads = 0;             // fused strided groups: tile width (0: default)
     std::vector<int32_t> widths;     // tile widths the tuner may choose from
@@ -86,6 +87,20 @@ int launch_group(paliwa_batch &b, paliwa_batch::Group const &grp,
     hipError_t const err = launch_fused_batched(
         b.dtype, p, grp.strided, ManyGrids{grp.grids_dev, grp.tile_grid_dev},
         grp.tiles, stream);
+    if (err != hipSuccess)
+      return fail(PALIWA_ERR_HIP, std::string("HIP: ") + hipGetErrorString(err));
+    return PALIWA_OK;
+  }
+  if (grp.long_ladder) {
+    SweepParams p{};
+    p.m = grp.m;
+    p.L = grp.L;
+    set_filters(p, filters, nfilters);
+    set_named_levels(p, grp.L, b.lmin[grp.dim], forward);
+    classify(p);
+    p.threads = grp.threads;
+    hipError_t const err = launch_long_ladder_batched(
+        b.dtype, p, grp.strided, ManyGrids{grp.grids_dev, grp.tile_grid_dev}, grp.tiles, stream);
     if (err != hipSuccess)
       return fail(PALIWA_ERR_HIP, std::string("HIP: ") + hipGetErrorString(err));
     return PALIWA_OK;
@@ -147,7 +162,7 @@ int launch_sequence(paliwa_batch &b, WaveletEntry const &w, bool forward,
       // run beside the large groups instead of after them
       for (int pass = 0; pass < 2 && rc == PALIWA_OK; ++pass)
         for (size_t k = i; k < j && rc == PALIWA_OK; ++k)
-          if (b.groups[k].lds_lines == (pass == 0))
+          if ((b.groups[k].lds_lines || b.groups[k].long_ladder) == (pass == 0))
             rc = launch_group(b, b.groups[k], filters, nfilters, forward,
                               b.lanes[b.lane_of[k]]);
       for (size_t t = 0; t < b.lanes.size() && rc == PALIWA_OK; ++t)
@@ -235,6 +250,8 @@ void drop_graphs(paliwa_batch *b) {
 
 // Lines per tile of a group at its current width.
 int64_t tile_lines(paliwa_batch const &b, paliwa_batch::Group const &grp) {
+  if (grp.long_ladder)
+    return long_tile_lines(b.dtype, grp.L, grp.threads);
   if (!grp.fused)
     return 1;
   int const G = (1 << grp.L) / fused_chunk(b.dtype, grp.L);
@@ -249,7 +266,7 @@ int build_tables(paliwa_batch *b) {
   for (size_t i = 0; i < b->groups.size(); ++i) {
     auto &grp = b->groups[i];
     grp.tiles = 0;
-    if (!grp.fused && !grp.lds_lines)
+    if (!grp.fused && !grp.long_ladder && !grp.lds_lines)
       continue;
     int64_t const tb = tile_lines(*b, grp);
     for (size_t k = 0; k < grp.members.size(); ++k) {
@@ -329,7 +346,8 @@ int setup_lanes(paliwa_batch *b) {
           pts += e;
         }
         int64_t const launch = b->dtype == PALIWA_FLOAT64 ? 3750000 : 7500000;
-        pts += b->groups[k].lds_lines ? b->long_weight * launch : launch;
+        bool const long_lines = b->groups[k].lds_lines || b->groups[k].long_ladder;
+        pts += long_lines ? b->long_weight * launch : launch;
         work.push_back({pts, k});
       }
       std::sort(work.begin(), work.end(), [](auto const &x, auto const &y) {
@@ -738,10 +756,22 @@ int paliwa_batch_create(paliwa_batch **plan, int32_t dtype, int32_t ndim,
       grp.strided = d + 1 < ndim;
       grp.members = members;
       grp.fused = (int64_t(1) << grp.L) <= fused_max_period(dtype);
-      grp.lds_lines = !grp.fused && (int64_t(1) << grp.L) <= lds_line_max_period(dtype);
       for (int32_t g : members) {
         SweepParams const q = make_params(b->data[g], ndim, b->extents_of(g), b->level_of(g), d);
         grp.lines += q.A * q.B;
+      }
+      // long lines: the long-line kernel on strided axes, the LDS-line kernel
+      // on the contiguous one as far as it holds the line (as the single-grid
+      // schedule chooses), the long-line kernel beyond
+      int64_t const n = int64_t(1) << grp.L;
+      grp.long_ladder = !grp.fused && n <= long_ladder_max_period(dtype) &&
+                        (grp.strided || n > lds_line_max_period(dtype));
+      grp.lds_lines = !grp.fused && !grp.long_ladder && n <= lds_line_max_period(dtype);
+      if (grp.long_ladder) {
+        grp.threads = long_threads_for(dtype, grp.L, grp.lines);
+        int const G = int(n / (dtype == PALIWA_FLOAT64 ? 32 : 64));
+        if (G <= 256)
+          grp.widths = {256, 1024};
       }
       if (grp.fused && grp.strided) {
         // the tile width of a strided float ladder follows the group's lines
@@ -794,7 +824,7 @@ int paliwa_batch_info(paliwa_batch const *plan, int32_t *launches,
     return fail(PALIWA_ERR_INVALID_ARGUMENT, "plan is NULL");
   int32_t n = 0;
   for (auto const &grp : plan->groups)
-    n += grp.fused ? 1 : int32_t(grp.members.size());
+    n += grp.fused || grp.long_ladder || grp.lds_lines ? 1 : int32_t(grp.members.size());
   int64_t total = 0;
   for (int32_t g = 0; g < plan->ngrids; ++g) {
     int64_t e = 1;

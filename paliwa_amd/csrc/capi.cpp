@@ -89,7 +89,7 @@ int paliwa_transform_in_path(void *data, int32_t dtype, int32_t ndim,
     return fail(PALIWA_ERR_INVALID_ARGUMENT, "levels is NULL");
   if ((rc = check_filters(filters, nfilters)) != PALIWA_OK)
     return rc;
-  if (path < PALIWA_PATH_AUTO || path > PALIWA_PATH_PER_PASS)
+  if (path < PALIWA_PATH_AUTO || path > PALIWA_PATH_LONG_LADDER)
     return fail(PALIWA_ERR_INVALID_ARGUMENT, "unknown kernel path");
   SweepParams p = make_params(data, ndim, extents, level, dim);
   p.nlevels = nlevels;

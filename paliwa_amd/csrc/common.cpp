@@ -132,6 +132,8 @@ int sweep(int32_t dtype, SweepParams p, int32_t path, hipStream_t stream) {
   if (path == PALIWA_PATH_AUTO) {
     if (n <= paliwa_hip::fused_max_period(dtype))
       path = PALIWA_PATH_FUSED;
+    else if (paliwa_hip::long_ladder_preferred(dtype, p))
+      path = PALIWA_PATH_LONG_LADDER;
     else if (n <= paliwa_hip::lds_line_max_period(dtype))
       path = PALIWA_PATH_LDS_LINE;
     else
@@ -151,6 +153,13 @@ int sweep(int32_t dtype, SweepParams p, int32_t path, hipStream_t stream) {
     break;
   case PALIWA_PATH_PER_PASS:
     err = paliwa_hip::launch_per_pass(dtype, p, stream);
+    break;
+  case PALIWA_PATH_LONG_LADDER:
+    if (!paliwa_hip::long_ladder_fits(dtype, p))
+      return fail(PALIWA_ERR_UNSUPPORTED,
+                  "the long-line kernel takes lines above the fused kernel's periods "
+                  "and consecutive level lists with the named tables' parities");
+    err = paliwa_hip::launch_long_ladder(dtype, p, stream);
     break;
   default:
     return fail(PALIWA_ERR_INVALID_ARGUMENT, "unknown kernel path");

@@ -189,4 +189,21 @@ hipError_t launch_lds_line_batched(int dtype, SweepParams const &p,
 hipError_t launch_per_pass(int dtype, SweepParams const &p,
                            hipStream_t stream);
 
+// Long lines (long_ladder.hip): periods above fused_max_period up to
+// long_ladder_max_period, ladder schedules only; tiles of 256 or 1024 threads
+// (long_threads_for: 256 where 1024-thread tiles would be fewer than the CUs
+// and a line fits 256 threads), long_tile_lines(dtype, L, threads) lines each.
+bool long_ladder_fits(int dtype, SweepParams const &p);
+// the automatic schedule's choice: strided axes, and contiguous lines longer
+// than the LDS-line kernel holds
+bool long_ladder_preferred(int dtype, SweepParams const &p);
+int long_ladder_max_period(int dtype);
+int long_threads_for(int dtype, int L, int64_t lines);
+int64_t long_tile_lines(int dtype, int L, int threads);
+hipError_t launch_long_ladder(int dtype, SweepParams const &p, hipStream_t stream);
+// Batched: `tiles` tiles of p.threads threads (256 or 1024) of the grids in
+// `grids`.
+hipError_t launch_long_ladder_batched(int dtype, SweepParams const &p, bool strided,
+                                      ManyGrids grids, int64_t tiles, hipStream_t stream);
+
 } // namespace paliwa_hip

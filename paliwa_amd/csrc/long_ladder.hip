@@ -1,0 +1,340 @@
+// Axis sweeps of long lines (periods beyond the fused register kernel):
+// float64 lines of 2^12 .. 2^15 points, float32 lines of 2^13 .. 2^16, for the
+// ladder schedules of the named transforms (hierarchize_in's and
+// dehierarchize_in's consecutive level lists with the reference's table
+// parities, paliwa/paliwa_transform.hpp:346-399).
+//
+// A workgroup of NT threads owns a tile of TB lines; each line of period n is
+// split into G = n / K chunks of K consecutive points, one chunk per thread in
+// registers (TB * G = NT).  Every point is read from HBM once and written once
+// per sweep:
+//   * levels whose storage stride s is below K run inside the chunk, in
+//     registers; each pass needs one value of a neighbouring chunk of the same
+//     line (the next chunk's first point for an odd pass, the previous chunk's
+//     point K - s for an even pass), exchanged through LDS with one barrier
+//     per pass (two slot arrays alternate, so a pass's writes never meet the
+//     previous pass's reads);
+//   * the coarser levels touch only the chunks' first points: those go to
+//     LDS, the remaining levels run there with one barrier per pass, and they
+//     come back to the registers.
+// Along a strided axis the lanes run across the tile's lines (the TB columns
+// of a row are consecutive in memory, so each row access is TB points wide);
+// along the contiguous axis they run along the line (every lane reads its K
+// consecutive points).  Each update is c0*g[lower] + c1*g[x] + c2*g[upper],
+// left to right without contraction, exactly the reference's expression, and
+// a pass is a pure function of the state before it (written and read points
+// have opposite parity), so the results are bit-identical to the reference's.
+#include "fused_sweep.hpp"
+#include "lifting.hpp"
+
+#include <algorithm>
+#include <type_traits>
+#include <utility>
+
+namespace paliwa_hip {
+namespace {
+
+constexpr int kLongThreads = 1024; // the widest tile; 256-thread tiles for few lines
+
+template <typename T> struct LongCoefs {
+  T c[2][3];
+};
+
+template <typename T>
+__device__ __forceinline__ T lift3(T c0, T lo, T c1, T x, T c2, T hi) {
+  return c0 * lo + c1 * x + c2 * hi;
+}
+
+// One pass at compile-time stride S (S < K) inside a chunk; `nb` is the
+// neighbouring chunk's value (next chunk's point 0 for odd passes, previous
+// chunk's point K - S for even passes).
+template <typename T, int K, int S, bool ODD>
+__device__ __forceinline__ void chunk_pass(T (&v)[K], T nb, T const (&c)[3]) {
+  if constexpr (ODD) {
+#pragma unroll
+    for (int j = S; j < K; j += 2 * S) {
+      T const hi = j + S < K ? v[j + S] : nb;
+      v[j] = lift3(c[0], v[j - S], c[1], v[j], c[2], hi);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < K; j += 2 * S) {
+      T const lo = j == 0 ? nb : v[j - S];
+      v[j] = lift3(c[0], lo, c[1], v[j], c[2], v[j + S]);
+    }
+  }
+}
+
+template <int PAT> constexpr int pat_passes() { return PAT == kPatOdd ? 1 : 2; }
+template <int PAT> constexpr bool pat_odd(int f) {
+  return PAT == kPatOdd ? true : (PAT == kPatOddEven ? f == 0 : f == 1);
+}
+
+// Geometry of the tile held by thread `tid`: line li of the tile, chunk gi.
+struct LongGeom {
+  int G, TB, li, gi;
+  int next, prev; // threads holding chunks gi + 1 and gi - 1 of the same line
+  __device__ LongGeom(int tid, int G_, int TB_, bool strided) : G(G_), TB(TB_) {
+    if (strided) {
+      li = tid % TB;
+      gi = tid / TB;
+    } else {
+      gi = tid % G;
+      li = tid / G;
+    }
+    int const gn = gi + 1 == G ? 0 : gi + 1;
+    int const gp = gi == 0 ? G - 1 : gi - 1;
+    next = strided ? li + TB * gn : gn + G * li;
+    prev = strided ? li + TB * gp : gp + G * li;
+  }
+};
+
+template <typename T, int K, int NT, bool STRIDED, int PAT, bool ASC, typename Src>
+__device__ __forceinline__ void long_tiles(SweepParams const &p, Src const &src,
+                                           int64_t ntiles) {
+  __shared__ T xch[2][NT];
+  __shared__ T heads[NT];
+  constexpr int LOGK = fused::ilog2(K);
+  int const tid = threadIdx.x;
+  int const n = 1 << p.L;
+  int const m = p.m;
+  int const G = n / K;
+  int const TB = NT / G;
+  LongGeom const geo(tid, G, TB, STRIDED);
+  LongCoefs<T> cf;
+#pragma unroll
+  for (int f = 0; f < pat_passes<PAT>(); ++f)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      cf.c[f][k] = static_cast<T>(p.coef[f][k]);
+  int buf = 0; // exchange slot array of the next fine pass
+
+  for (int64_t t = fused::xcd_tile(blockIdx.x, gridDim.x); t < ntiles; t += gridDim.x) {
+    TileRef const ref = src(t);
+    int64_t const line = ref.local * TB + geo.li;
+    bool const active = line < ref.A * ref.B;
+    T *base = nullptr;
+    int64_t step = 1; // distance between consecutive points of the line
+    if (active) {
+      int64_t const a = line / ref.B;
+      int64_t const b = line - a * ref.B;
+      base = static_cast<T *>(ref.data) + a * int64_t(m) * ref.B + b;
+      step = ref.B;
+    }
+    T v[K];
+    if (active) {
+      T const *q = base + int64_t(geo.gi) * K * step;
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+        v[j] = q[j * step];
+    } else {
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+        v[j] = T(0);
+    }
+
+    // fine levels (stride 2^e < K) in registers
+    auto fine_level = [&](auto e_tag) {
+      constexpr int E = decltype(e_tag)::value;
+      constexpr int S = 1 << E;
+#pragma unroll
+      for (int f = 0; f < pat_passes<PAT>(); ++f) {
+        bool const odd = pat_odd<PAT>(f);
+        xch[buf][tid] = odd ? v[0] : v[K - S];
+        __syncthreads();
+        T const nb = xch[buf][odd ? geo.next : geo.prev];
+        buf ^= 1;
+        if (odd)
+          chunk_pass<T, K, S, true>(v, nb, cf.c[f]);
+        else
+          chunk_pass<T, K, S, false>(v, nb, cf.c[f]);
+      }
+    };
+    // the fine levels in the order of the tags (uniform conditions: every
+    // thread of the workgroup takes the same branches to the barriers)
+    auto fine_levels = [&](auto... e_tags) {
+      ((decltype(e_tags)::value >= p.lo && decltype(e_tags)::value <= p.hi
+            ? fine_level(e_tags)
+            : void()),
+       ...);
+    };
+    // coarse levels (stride 2^e >= K) on the chunks' first points, in LDS
+    auto coarse_levels = [&]() {
+      int const e0 = p.lo > LOGK ? p.lo : LOGK;
+      if (p.hi < e0)
+        return;
+      heads[tid] = v[0];
+      __syncthreads();
+      for (int i = 0; i <= p.hi - e0; ++i) {
+        int const e = ASC ? e0 + i : p.hi - i;
+        int const S = 1 << (e - LOGK); // in chunks
+#pragma unroll
+        for (int f = 0; f < pat_passes<PAT>(); ++f) {
+          int const off = pat_odd<PAT>(f) ? S : 0;
+          if ((geo.gi & (2 * S - 1)) == off) {
+            int const gl = geo.gi - S < 0 ? geo.gi - S + G : geo.gi - S;
+            int const gh = geo.gi + S >= G ? geo.gi + S - G : geo.gi + S;
+            int const tl = STRIDED ? geo.li + TB * gl : gl + G * geo.li;
+            int const th = STRIDED ? geo.li + TB * gh : gh + G * geo.li;
+            heads[tid] = lift3(cf.c[f][0], heads[tl], cf.c[f][1], heads[tid], cf.c[f][2],
+                               heads[th]);
+          }
+          __syncthreads();
+        }
+      }
+      v[0] = heads[tid];
+      __syncthreads(); // heads may be rewritten by the next tile
+    };
+    if constexpr (ASC) {
+      [&]<int... E>(std::integer_sequence<int, E...>) {
+        fine_levels(std::integral_constant<int, E>{}...);
+      }(std::make_integer_sequence<int, LOGK>{});
+      coarse_levels();
+    } else {
+      coarse_levels();
+      [&]<int... E>(std::integer_sequence<int, E...>) {
+        fine_levels(std::integral_constant<int, LOGK - 1 - E>{}...);
+      }(std::make_integer_sequence<int, LOGK>{});
+    }
+
+    if (active) {
+      T *q = base + int64_t(geo.gi) * K * step;
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+        q[j * step] = v[j];
+      if (m != n && geo.gi == 0)
+        base[int64_t(n) * step] = v[0];
+    }
+  }
+}
+
+template <typename T, int K, int NT, bool STRIDED, int PAT, bool ASC>
+__global__ __launch_bounds__(NT) void long_ladder(SweepParams p, int64_t ntiles) {
+  long_tiles<T, K, NT, STRIDED, PAT, ASC>(p, OneGrid{p.data, p.A, p.B}, ntiles);
+}
+
+template <typename T, int K, int NT, bool STRIDED, int PAT, bool ASC>
+__global__ __launch_bounds__(NT) void long_ladder_batched(SweepParams p, int64_t ntiles,
+                                                          ManyGrids grids) {
+  long_tiles<T, K, NT, STRIDED, PAT, ASC>(p, grids, ntiles);
+}
+
+template <typename T> constexpr int long_chunk() { return sizeof(T) == 8 ? 32 : 64; }
+
+template <int NT, typename Kernel, typename... Extra>
+hipError_t launch_long_kernel(Kernel kernel, SweepParams const &p, int64_t tiles,
+                              hipStream_t stream, Extra... extra) {
+  int64_t const resident =
+      fused::resident_workgroups(reinterpret_cast<const void *>(kernel), 0, NT);
+  uint32_t const grid = static_cast<uint32_t>(tiles < resident ? tiles : resident);
+  if (grid == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(NT), 0, stream, p, tiles, extra...);
+  return hipGetLastError();
+}
+
+template <typename T, bool STRIDED, bool BATCHED, int NT, typename... Extra>
+hipError_t dispatch_long_nt(SweepParams const &p, int64_t tiles, hipStream_t stream,
+                            Extra... extra) {
+  constexpr int K = long_chunk<T>();
+  bool const asc = p.schedule == kScheduleLadderAsc;
+  auto go = [&](auto pat, auto asc_tag) -> hipError_t {
+    constexpr int PAT = decltype(pat)::value;
+    constexpr bool ASC = decltype(asc_tag)::value;
+    if constexpr (BATCHED)
+      return launch_long_kernel<NT>(long_ladder_batched<T, K, NT, STRIDED, PAT, ASC>, p,
+                                    tiles, stream, extra...);
+    else
+      return launch_long_kernel<NT>(long_ladder<T, K, NT, STRIDED, PAT, ASC>, p, tiles,
+                                    stream);
+  };
+  using F = std::false_type;
+  using Tr = std::true_type;
+  switch (p.pattern) {
+  case kPatOdd:
+    return asc ? go(std::integral_constant<int, kPatOdd>{}, Tr{})
+               : go(std::integral_constant<int, kPatOdd>{}, F{});
+  case kPatOddEven:
+    return asc ? go(std::integral_constant<int, kPatOddEven>{}, Tr{})
+               : go(std::integral_constant<int, kPatOddEven>{}, F{});
+  case kPatEvenOdd:
+    return asc ? go(std::integral_constant<int, kPatEvenOdd>{}, Tr{})
+               : go(std::integral_constant<int, kPatEvenOdd>{}, F{});
+  default:
+    return hipErrorInvalidValue;
+  }
+}
+
+// A line of G chunks fits a 256-thread tile when G <= 256.
+template <typename T, bool STRIDED, bool BATCHED, typename... Extra>
+hipError_t dispatch_long(SweepParams const &p, int threads, int64_t tiles, hipStream_t stream,
+                         Extra... extra) {
+  if (threads == 256)
+    return dispatch_long_nt<T, STRIDED, BATCHED, 256>(p, tiles, stream, extra...);
+  return dispatch_long_nt<T, STRIDED, BATCHED, kLongThreads>(p, tiles, stream, extra...);
+}
+
+template <typename T>
+hipError_t launch_long_typed(SweepParams const &p, hipStream_t stream) {
+  int const dtype = sizeof(T) == 8 ? PALIWA_FLOAT64 : PALIWA_FLOAT32;
+  int64_t const lines = p.A * p.B;
+  int const threads = long_threads_for(dtype, p.L, lines);
+  int64_t const tb = long_tile_lines(dtype, p.L, threads);
+  int64_t const tiles = (lines + tb - 1) / tb;
+  return p.B > 1 ? dispatch_long<T, true, false>(p, threads, tiles, stream)
+                 : dispatch_long<T, false, false>(p, threads, tiles, stream);
+}
+
+} // namespace
+
+bool long_ladder_fits(int dtype, SweepParams const &p) {
+  int64_t const n = int64_t(1) << p.L;
+  bool const ladder = (p.schedule == kScheduleLadderAsc || p.schedule == kScheduleLadderDesc) &&
+                      p.pattern != kPatNone;
+  return ladder && n > fused_max_period(dtype) && n <= long_ladder_max_period(dtype);
+}
+
+bool long_ladder_preferred(int dtype, SweepParams const &p) {
+  // along the contiguous axis a lane reads its own K consecutive points, which
+  // no load instruction coalesces: the LDS-line kernel, which stages a whole
+  // line with coalesced copies, is faster there (4097^2 float64: 110 against
+  // 131 us) for as long as it holds the line
+  return long_ladder_fits(dtype, p) &&
+         (p.B > 1 || (int64_t(1) << p.L) > lds_line_max_period(dtype));
+}
+
+int long_ladder_max_period(int dtype) {
+  return dtype == PALIWA_FLOAT64 ? 32 * kLongThreads : 64 * kLongThreads;
+}
+
+int long_threads_for(int dtype, int L, int64_t lines) {
+  int64_t const G = (int64_t(1) << L) / (dtype == PALIWA_FLOAT64 ? 32 : 64);
+  if (G > 256)
+    return kLongThreads;
+  // 1024-thread tiles of few lines leave CUs idle: 256-thread tiles (4 times
+  // as many workgroups) where the wide ones would be fewer than the 256 CUs
+  int64_t const wide_tiles = (lines * G + kLongThreads - 1) / kLongThreads;
+  return wide_tiles < 256 ? 256 : kLongThreads;
+}
+
+int64_t long_tile_lines(int dtype, int L, int threads) {
+  int const K = dtype == PALIWA_FLOAT64 ? 32 : 64;
+  return threads / ((int64_t(1) << L) / K);
+}
+
+hipError_t launch_long_ladder(int dtype, SweepParams const &p, hipStream_t stream) {
+  return dtype == PALIWA_FLOAT64 ? launch_long_typed<double>(p, stream)
+                                 : launch_long_typed<float>(p, stream);
+}
+
+hipError_t launch_long_ladder_batched(int dtype, SweepParams const &p, bool strided,
+                                      ManyGrids grids, int64_t tiles, hipStream_t stream) {
+  int const threads = p.threads == 256 ? 256 : kLongThreads;
+  if (dtype == PALIWA_FLOAT64)
+    return strided ? dispatch_long<double, true, true>(p, threads, tiles, stream, grids)
+                   : dispatch_long<double, false, true>(p, threads, tiles, stream, grids);
+  return strided ? dispatch_long<float, true, true>(p, threads, tiles, stream, grids)
+                 : dispatch_long<float, false, true>(p, threads, tiles, stream, grids);
+}
+
+} // namespace paliwa_hip

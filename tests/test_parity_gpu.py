@@ -243,6 +243,76 @@ def test_long_lines_lds_and_per_pass(pw):
             assert_same(got, oracle_named(a, level, lmin, name, True), f"L={L} {extents}")
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_long_ladder_every_period(pw, dtype):
+    """The long-line kernel (lines beyond the fused kernel: float64 2^12..2^15,
+    float32 2^13..2^16 points) on strided axes (3 and 33 columns, several
+    tiles of lines with a partial last one) and the contiguous axis, with and
+    without images, every wavelet, both directions, three minimum levels;
+    the explicit path equals the LDS-line path where that one holds the line."""
+    rng = np.random.default_rng(18)
+    periods = range(12, 16) if dtype == np.float64 else range(13, 17)
+    for L in periods:
+        n = 1 << L
+        for extents, level, dim in (([n + 1, 3], [L, 1], 0), ([n, 33], [L, 5], 0),
+                                    ([3, n + 1], [1, L], 1), ([2, n], [1, L], 1)):
+            for name in WAVELETS:
+                lmin_L = {12: 0, 13: 6, 14: 13, 15: 0, 16: 9}[L]
+                lmin = [lmin_L if d == dim else 0 for d in range(2)]
+                a = closed_random(rng, extents, level, dtype)
+                for fwd in (True, False):
+                    got = gpu_named(pw, a, level, lmin, name, fwd)
+                    want = oracle_named(a, level, lmin, name, fwd)
+                    assert_same(got, want, f"L={L} {extents} {name} lmin={lmin} fwd={fwd}")
+        # the explicit schedule, and the LDS-line kernel on the same line
+        level, extents = [L, 2], [n + 1, 5]
+        levels = list(range(L, 0, -1))
+        a = closed_random(rng, extents, level, dtype)
+        out = {}
+        for path in ("long_ladder", "lds_line") if L <= (14 if dtype == np.float64 else 15) \
+                else ("long_ladder",):
+            t = torch.from_numpy(a).cuda()
+            pw.transform_in(0, t, level, level, levels, ob.FORWARD["biorthogonal"], path=path)
+            out[path] = t.cpu().numpy()
+        want = a.copy()
+        ob.transform_in(want, level, 0, levels, ob.FORWARD["biorthogonal"])
+        for path, got in out.items():
+            assert_same(got, want, f"{path} L={L}")
+
+
+@pytest.mark.parametrize("dtype,L", [(np.float64, 12), (np.float32, 13)])
+def test_long_ladder_wide_tiles(pw, dtype, L):
+    """More than 2048 strided lines of 2^12 (float64) / 2^13 (float32) points
+    take 1024-thread tiles (the tests above, with few lines, take 256-thread
+    ones): both directions against the oracle."""
+    rng = np.random.default_rng(20)
+    n = 1 << L
+    level, extents = [L, 11], [n + 1, 2049]
+    for name in ("hat", "biorthogonal"):
+        a = closed_random(rng, extents, level, dtype)
+        lmin = [0, 11]
+        fwd = gpu_named(pw, a, level, lmin, name, True)
+        assert_same(fwd, oracle_named(a, level, lmin, name, True), f"{extents} {name} fwd")
+        back = gpu_named(pw, fwd, level, lmin, name, False)
+        assert_same(back, oracle_named(fwd, level, lmin, name, False), f"{extents} {name} inv")
+
+
+def test_long_ladder_rejects_generic_lists(pw):
+    """The long-line kernel runs ladders only: an explicit request with a
+    non-consecutive level list is refused; the automatic schedule takes the
+    LDS-line kernel for it, bit-exact."""
+    rng = np.random.default_rng(19)
+    level, extents = [12, 2], [4097, 5]
+    a = closed_random(rng, extents, level, np.float64)
+    t = torch.from_numpy(a).cuda()
+    with pytest.raises(pw.UnsupportedError):
+        pw.transform_in(0, t, level, level, [12, 10, 11], ob.FORWARD["hat"], path="long_ladder")
+    pw.transform_in(0, t, level, level, [12, 10, 11], ob.FORWARD["hat"])
+    want = a.copy()
+    ob.transform_in(want, level, 0, [12, 10, 11], ob.FORWARD["hat"])
+    assert_same(t.cpu().numpy(), want, "generic list on a long line")
+
+
 def test_axes_without_levels(pw):
     """Axes whose minimum level equals their level sweep nothing: a grid
     with every axis so, or of level 0 only, comes back unchanged, and mixed
