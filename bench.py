@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--ct-only", action="store_true",
                     help="after the timed steps, only the combination-technique section "
                          "(counter runs of the batched plan and the combine)")
+    ap.add_argument("--ct-replays", type=int, default=0,
+                    help="only build the CT set's batched plan and replay its forward "
+                         "direction this many times (kernel traces of the schedule)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     return ap.parse_args()
@@ -354,6 +357,45 @@ def ct_benchmark(torch, pw, fields, reps=5):
     }
 
 
+def ct_set_grids(torch, fields):
+    """The CT set of the bench: 1325 4-D float64 grids with cosine-product
+    fields (seed 2)."""
+    import math
+
+    import numpy as np
+    import paliwa_amd as pw
+
+    levels = pw.combination_levels(4, (13, 14, 15, 16), 1)
+    rng = np.random.default_rng(2)
+    grids = []
+    for l in levels:
+        a = rng.uniform(1.0, 4.0, size=4)
+        phi = rng.uniform(0.0, 2 * math.pi, size=4)
+        grids.append(fields.cosine_product(l, a, phi, device="cuda"))
+    return levels, grids
+
+
+def ct_replays(torch, count):
+    """The batched plan of the CT set (rule-based settings), forward
+    direction replayed `count` times after two warm-up replays."""
+    import paliwa_amd as pw
+    from paliwa_amd import fields
+
+    levels, grids = ct_set_grids(torch, fields)
+    batch = pw.CombinationBatch(grids, levels, [0] * 4, [13] * 4)
+    for _ in range(2):
+        batch.hierarchize("hat")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(count):
+        batch.hierarchize("hat")
+    torch.cuda.synchronize()
+    print(json.dumps({"ct_replays": count,
+                      "ms_per_replay": round((time.perf_counter() - t0) * 1e3 / count, 4),
+                      "launches_per_replay": batch.info()["launches_per_direction"]}))
+    batch.close()
+
+
 def cpp_per_grid_loop():
     """The reference's per-grid loop in C++ through the drop-in header
     (tests/cpp/ct_per_grid.cpp): one stream, and 32 streams round-robin as the
@@ -537,6 +579,9 @@ def main():
         sys.exit(1)
     if args.copy_probe:
         copy_probe(torch)
+        return
+    if args.ct_replays:
+        ct_replays(torch, args.ct_replays)
         return
 
     import paliwa_amd as pw

@@ -4,8 +4,10 @@
  * (PALIWA_OK or a negative PALIWA_ERR_* code with a message available from
  * paliwa_last_error()), is stream-ordered and asynchronous on `stream` (a
  * hipStream_t; NULL is the default stream), never synchronizes the device and
- * allocates no device memory (paliwa_batch_create alone allocates, once, the
- * descriptor tables of its plan).  Arguments are validated on the host before
+ * allocates no device memory (the batched plans' set-up calls alone allocate:
+ * paliwa_batch_create and paliwa_batch_set_boxes the descriptor tables of a
+ * plan, paliwa_batch_tune a copy of its grids for the duration of the call).
+ * Arguments are validated on the host before
  * any HIP call, so a rejected call has no effect on the device.
  *
  * Data layout: a dense row-major array (last axis contiguous, as DDC's
@@ -260,6 +262,27 @@ int paliwa_batch_dehierarchize(paliwa_batch *plan, const char *wavelet_name,
 /* 0: launch the grouped kernels directly instead of replaying a graph. */
 int paliwa_batch_set_graphs(paliwa_batch *plan, int32_t enable);
 
+/* Box schedule (no counterpart in the reference).  Instead of one launch per
+ * group of axis sweeps, each grid's axes split into boxes of consecutive
+ * axes a..b: the full extents of those axes for one index of the axes before
+ * a and a tile of `width` consecutive indices of the flattened axes after b
+ * (all of them when the box then fits).  A box is swept in LDS, every axis of
+ * it with all its levels, so it costs one HBM read and one write for its axes
+ * together; launch p sweeps the p-th box of every grid.  Each grid takes the
+ * fewest boxes that fit `capacity_bytes` of LDS (40960, 81920 or 163840: four,
+ * two or one box per CU).  width: a power of two in [1, 4096]; threads: 256
+ * or 512 per box.  capacity_bytes 0: the per-axis groups.  Rebuilds the
+ * plan's descriptor tables (allocation, synchronous).  A new plan takes
+ * boxes of 40960 bytes with 512-byte tile rows (256 threads for float64, 512
+ * for float32) when they read at most three quarters of what the groups
+ * read.  Results are bit-identical
+ * either way.  Fails, leaving the plan as it was, when an axis of some grid
+ * does not fit a box alone. */
+int paliwa_batch_set_boxes(paliwa_batch *plan, int32_t capacity_bytes,
+                           int32_t width, int32_t threads);
+int paliwa_batch_get_boxes(const paliwa_batch *plan, int32_t *capacity_bytes,
+                           int32_t *width, int32_t *threads);
+
 /* Kernel launches per direction, total stored points, descriptor bytes. */
 int paliwa_batch_info(const paliwa_batch *plan, int32_t *launches,
                       int64_t *points, int64_t *descriptor_bytes);
@@ -271,8 +294,12 @@ int paliwa_batch_destroy(paliwa_batch *plan);
  * the plan's own grids and keeps the fastest, by coordinate descent on the
  * whole direction's time (median of 15 replays, a choice kept when at least
  * 1% faster):
- *   - the tile width (256 / 512 / 1024 threads, where the tile fits in LDS) of
- *     every strided group whose lines hold 512 points or more;
+ *   - the box schedule (paliwa_batch_set_boxes): none, or each LDS capacity
+ *     with tile rows of 128, 256 or 512 bytes and 256 or 512 threads (unless
+ *     flags has PALIWA_TUNE_KEEP_BOXES: the plan's setting stays);
+ *   - then, for the groups of that schedule, the tile width (256 / 512 / 1024
+ *     threads, where the tile fits in LDS) of every strided group whose lines
+ *     hold 512 points or more;
  *   - the stream schedule: 2, 3, 4, 6 or 8 streams for the groups of an axis,
  *     and the launch-cost weight (1, 4, 16) of long-line groups in the
  *     balance across them.
@@ -282,12 +309,14 @@ int paliwa_batch_destroy(paliwa_batch *plan);
  * depend on any choice.  cache_path (may be NULL): a text file of earlier
  * choices, one per line ("<key> = <value>", keys naming dtype, layout, line
  * period, stored extent, line-count bucket, wavelet and direction for widths,
- * a hash of the plan's groups for the schedule); choices found there are
+ * a hash of the plan's groups for the schedule, a hash of the grid shapes
+ * for the box schedule); choices found there are
  * applied without timing and the file is rewritten with every choice of this
  * plan.  When the cache holds every choice nothing is timed or allocated
  * unless flags has PALIWA_TUNE_MEASURE. */
 #define PALIWA_TUNE_FORCE 1   /* ignore the cache: time every choice */
 #define PALIWA_TUNE_MEASURE 2 /* time the rule-based and final settings */
+#define PALIWA_TUNE_KEEP_BOXES 4 /* keep the plan's box schedule */
 int paliwa_batch_tune(paliwa_batch *plan, const char *wavelet_name,
                       int32_t forward, const char *cache_path, int32_t flags,
                       void *stream);

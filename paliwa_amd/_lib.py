@@ -35,6 +35,7 @@ PATHS = {"auto": 0, "fused": 1, "lds_line": 2, "per_pass": 3, "long_ladder": 4}
 
 PALIWA_TUNE_FORCE = 1
 PALIWA_TUNE_MEASURE = 2
+PALIWA_TUNE_KEEP_BOXES = 4
 
 
 class paliwa_filter(ctypes.Structure):
@@ -120,6 +121,8 @@ SIGNATURES = {
     "paliwa_batch_dehierarchize": (ctypes.c_int, [_vp, ctypes.c_char_p, _vp]),
     "paliwa_batch_set_graphs": (ctypes.c_int, [_vp, _i32]),
     "paliwa_batch_info": (ctypes.c_int, [_vp, _i32p, _i64p, _i64p]),
+    "paliwa_batch_set_boxes": (ctypes.c_int, [_vp, _i32, _i32, _i32]),
+    "paliwa_batch_get_boxes": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p]),
     "paliwa_batch_destroy": (ctypes.c_int, [_vp]),
     "paliwa_batch_tune": (ctypes.c_int, [_vp, ctypes.c_char_p, _i32, ctypes.c_char_p, _i32, _vp]),
     "paliwa_batch_tune_info": (

@@ -77,6 +77,23 @@ class CombinationBatch:
     def use_graphs(self, enable: bool) -> None:
         check(_lib.lib.paliwa_batch_set_graphs(self._plan, int(bool(enable))))
 
+    def set_boxes(self, capacity_bytes: int, width: int = 32, threads: int = 256) -> None:
+        """Box schedule: each grid's axes in boxes of consecutive axes swept
+        in LDS (capacity 40960, 81920 or 163840 bytes; ``width`` inner points
+        per tile, a power of two; 256 or 512 threads); capacity 0 runs the
+        per-axis groups.  See paliwa_batch_set_boxes."""
+        with torch.cuda.device(self.device):
+            check(_lib.lib.paliwa_batch_set_boxes(self._plan, int(capacity_bytes), int(width),
+                                                  int(threads)))
+
+    def boxes(self):
+        """(capacity_bytes, width, threads) of the box schedule; capacity 0
+        when the plan runs its per-axis groups."""
+        cap, width, threads = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
+        check(_lib.lib.paliwa_batch_get_boxes(self._plan, ctypes.byref(cap), ctypes.byref(width),
+                                              ctypes.byref(threads)))
+        return cap.value, width.value, threads.value
+
     def info(self):
         launches = ctypes.c_int32(0)
         points = ctypes.c_int64(0)
@@ -87,13 +104,16 @@ class CombinationBatch:
                 "descriptor_bytes": nbytes.value}
 
     def tune(self, wavelet_name: str = "hat", forward: bool = True, cache=None,
-             force: bool = False, measure: bool = False, stream=None):
-        """Time alternative tile widths and stream schedules on the plan's own
-        grids (restored afterwards) and keep the fastest; see
-        paliwa_batch_tune.  ``cache``: path of a tuning cache file.  Returns
+             force: bool = False, measure: bool = False, keep_boxes: bool = False,
+             stream=None):
+        """Time alternative box schedules, tile widths and stream schedules on
+        the plan's own grids (restored afterwards) and keep the fastest; see
+        paliwa_batch_tune.  ``cache``: path of a tuning cache file;
+        ``keep_boxes``: leave the plan's box schedule as it is.  Returns
         {"rule_ms", "tuned_ms", "evaluations", "streams"}."""
         flags = (_lib.PALIWA_TUNE_FORCE if force else 0) | (
-            _lib.PALIWA_TUNE_MEASURE if measure else 0)
+            _lib.PALIWA_TUNE_MEASURE if measure else 0) | (
+            _lib.PALIWA_TUNE_KEEP_BOXES if keep_boxes else 0)
         with torch.cuda.device(self.device):
             check(_lib.lib.paliwa_batch_tune(
                 self._plan, str(wavelet_name).encode(), int(bool(forward)),

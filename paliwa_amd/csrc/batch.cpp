@@ -13,6 +13,7 @@
 #include "lifting.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
@@ -20,10 +21,22 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
 using namespace paliwa_hip;
+
+// Axes a..z of one grid as a box (box.hip): `outer` indices of the axes
+// before a, the full extents of a..z (`rows` points), tiles of `width` of the
+// `inner` points after z.  A box whose tile holds all inner points is one
+// contiguous run of the grid; `batch` consecutive outer indices of such boxes
+// share a workgroup.
+struct BoxShape {
+  int32_t a = 0, z = 0;
+  int64_t outer = 1, rows = 1, inner = 1;
+  int64_t width = 1, tiles = 1, batch = 1;
+};
 
 struct paliwa_batch {
   int device = 0;
@@ -34,7 +47,13 @@ struct paliwa_batch {
   std::vector<void *> data;
 
   struct Group {
+    int32_t phase = 0; // groups of one phase touch disjoint grids
     int32_t dim = 0, L = 0, m = 0;
+    bool box = false;  // the boxes of this phase: one box launch
+    std::vector<std::pair<int32_t, BoxShape>> boxes; // (grid, box)
+    int64_t box_blocks = 0;
+    BoxGrid *box_grids_dev = nullptr;
+    int32_t *box_block_grid_dev = nullptr;
     bool strided = false;
     bool fused = false;
     bool long_ladder = false;        // longer lines: one batched long-line launch
@@ -47,7 +66,7 @@ struct paliwa_batch {
     int32_t *tile_grid_dev = nullptr;
     std::vector<int32_t> members;    // grid indices
   };
-  std::vector<Group> groups; // in axis order
+  std::vector<Group> groups; // in phase order
   void *dev = nullptr;       // one allocation holding every descriptor table
   size_t dev_bytes = 0;
 
@@ -67,6 +86,16 @@ struct paliwa_batch {
   double rule_ms = -1.0, tuned_ms = -1.0;
   int32_t evaluations = 0;
 
+  // Box schedule (box.hip) when box_capacity > 0: each grid's axes split
+  // into segments swept in turn, boxes of consecutive axes that fit the LDS
+  // capacity or single axes through the groups' kernels; phase p holds the
+  // p-th segment of every grid.  Without boxes phase p is axis p.
+  int32_t box_capacity = 0; // bytes of LDS per box (kBoxCapacities), 0: no boxes
+  int32_t box_width = 0;    // inner points per tile of a tiled box (power of two)
+  int32_t box_threads = 256;
+  void *box_dev = nullptr;  // the box groups' descriptor tables
+  size_t box_dev_bytes = 0;
+
   int32_t const *level_of(int g) const { return &level[size_t(g) * ndim]; }
   int64_t const *extents_of(int g) const { return &extents[size_t(g) * ndim]; }
 };
@@ -76,6 +105,26 @@ namespace {
 int launch_group(paliwa_batch &b, paliwa_batch::Group const &grp,
                  paliwa_filter const *filters, int nfilters, bool forward,
                  hipStream_t stream) {
+  if (grp.box) {
+    if (nfilters < 1 || nfilters > 2)
+      return fail(PALIWA_ERR_UNSUPPORTED, "boxes run filter tables of one or two passes");
+    BoxLaunch launch{};
+    launch.grids = grp.box_grids_dev;
+    launch.block_grid = grp.box_block_grid_dev;
+    launch.blocks = grp.box_blocks;
+    launch.forward = forward;
+    launch.npass = nfilters;
+    for (int f = 0; f < nfilters; ++f) {
+      launch.offset[f] = filters[f].offset;
+      for (int c = 0; c < 3; ++c)
+        launch.coef[f][c] = filters[f].coefficients[c];
+    }
+    hipError_t const err =
+        launch_box_batched(b.dtype, launch, b.box_capacity, b.box_threads, stream);
+    if (err != hipSuccess)
+      return fail(PALIWA_ERR_HIP, std::string("HIP: ") + hipGetErrorString(err));
+    return PALIWA_OK;
+  }
   if (grp.fused) {
     SweepParams p{};
     p.m = grp.m;
@@ -140,9 +189,9 @@ int launch_sequence(paliwa_batch &b, WaveletEntry const &w, bool forward,
   };
   size_t i = 0;
   while (i < b.groups.size()) {
-    // the groups of one axis
+    // the groups of one phase
     size_t j = i;
-    while (j < b.groups.size() && b.groups[j].dim == b.groups[i].dim)
+    while (j < b.groups.size() && b.groups[j].phase == b.groups[i].phase)
       ++j;
     if (j - i == 1 || b.lanes.empty()) {
       for (size_t k = i; k < j; ++k) {
@@ -318,7 +367,224 @@ int build_tables(paliwa_batch *b) {
   return PALIWA_OK;
 }
 
+// ---- box schedule --------------------------------------------------------
+
+// The box of axes a..z of grid g within `cap` points, or false: the tile holds
+// all inner points when they fit (several consecutive outer indices per
+// workgroup then, up to the capacity), else `width` of them.
+bool make_box(paliwa_batch const &b, int32_t g, int32_t a, int32_t z, int64_t cap,
+              int64_t width, BoxShape *out) {
+  int64_t const *e = b.extents_of(g);
+  BoxShape x;
+  x.a = a;
+  x.z = z;
+  for (int32_t d = 0; d < a; ++d)
+    x.outer *= e[d];
+  for (int32_t d = a; d <= z; ++d)
+    x.rows *= e[d];
+  for (int32_t d = z + 1; d < b.ndim; ++d)
+    x.inner *= e[d];
+  // a box's span in its grid stays below 2^28 points: 32-bit offsets
+  if (z - a + 1 > kBoxMaxAxes || x.rows > cap || x.rows * x.inner >= (int64_t(1) << 28))
+    return false;
+  if (x.rows * x.inner <= cap) {
+    x.width = x.inner;
+    x.batch = std::min(x.outer, cap / (x.rows * x.inner));
+  } else if (x.rows * width <= cap && width < x.inner) {
+    x.width = width;
+  } else {
+    return false;
+  }
+  x.tiles = (x.inner + x.width - 1) / x.width;
+  *out = x;
+  return true;
+}
+
+// One sweep of a grid's schedule: a box of axes a..z, or the single axis a
+// through the groups' kernels.
+struct Segment {
+  bool box = false;
+  BoxShape shape;
+};
+
+// The segments of grid g in sweep order.  Without boxes: every axis with
+// levels to sweep.  With boxes: the fewest segments; among as many, the
+// fewest single-axis boxes (a single axis runs in the groups' register
+// kernels), then the fewest points in the empty part of partial tiles.
+// Segments with no levels to sweep are left out.
+std::vector<Segment> segments_of(paliwa_batch const &b, int32_t g) {
+  int32_t const D = b.ndim;
+  int32_t const *lv = b.level_of(g);
+  auto swept = [&](int32_t a, int32_t z) {
+    for (int32_t d = a; d <= z; ++d)
+      if (lv[d] > b.lmin[d])
+        return true;
+    return false;
+  };
+  std::vector<Segment> out;
+  if (b.box_capacity == 0) {
+    for (int32_t d = 0; d < D; ++d)
+      if (swept(d, d)) {
+        Segment s;
+        s.shape.a = s.shape.z = d;
+        out.push_back(s);
+      }
+    return out;
+  }
+  int64_t const cap = box_points(b.box_capacity, b.dtype == PALIWA_FLOAT64 ? 8 : 4);
+  struct Best {
+    bool ok = false;
+    int32_t segments = 0, single_boxes = 0;
+    int64_t waste = 0;
+    int32_t from = 0;
+    Segment last;
+  };
+  std::vector<Best> best(size_t(D) + 1);
+  best[0].ok = true;
+  for (int32_t z = 0; z < D; ++z)
+    for (int32_t a = 0; a <= z; ++a) {
+      if (!best[a].ok)
+        continue;
+      auto offer = [&](Segment const &seg, int64_t waste) {
+        Best const &f = best[a];
+        Best c{true, f.segments + 1, f.single_boxes + (seg.box && a == z ? 1 : 0),
+               f.waste + waste, a, seg};
+        Best &t = best[size_t(z) + 1];
+        if (!t.ok || std::tie(c.segments, c.single_boxes, c.waste) <
+                         std::tie(t.segments, t.single_boxes, t.waste))
+          t = c;
+      };
+      Segment seg;
+      seg.box = true;
+      if (make_box(b, g, a, z, cap, b.box_width, &seg.shape))
+        offer(seg, seg.shape.outer * seg.shape.rows *
+                       (seg.shape.tiles * seg.shape.width - seg.shape.inner));
+      if (a == z) {
+        Segment one;
+        one.shape.a = one.shape.z = a;
+        offer(one, 0);
+      }
+    }
+  for (int32_t z = D; z > 0; z = best[z].from)
+    out.push_back(best[z].last);
+  std::reverse(out.begin(), out.end());
+  out.erase(std::remove_if(out.begin(), out.end(),
+                           [&](Segment const &x) { return !swept(x.shape.a, x.shape.z); }),
+            out.end());
+  return out;
+}
+
+void drop_box_tables(paliwa_batch *b) {
+  if (b->box_dev != nullptr)
+    (void)hipFree(b->box_dev);
+  b->box_dev = nullptr;
+  b->box_dev_bytes = 0;
+  for (auto &grp : b->groups) {
+    grp.box_grids_dev = nullptr;
+    grp.box_block_grid_dev = nullptr;
+  }
+}
+
+// Descriptor tables of the box groups: one device allocation.
+int build_box_tables(paliwa_batch *b) {
+  drop_box_tables(b);
+  std::vector<std::vector<BoxGrid>> grids(b->groups.size());
+  std::vector<std::vector<int32_t>> block_grid(b->groups.size());
+  for (size_t i = 0; i < b->groups.size(); ++i) {
+    auto &grp = b->groups[i];
+    if (!grp.box)
+      continue;
+    grp.box_blocks = 0;
+    for (auto const &[g, x] : grp.boxes) {
+      BoxGrid d{};
+      d.data = b->data[g];
+      d.outer_stride = x.rows * x.inner;
+      d.inner = x.inner;
+      d.outer = x.outer;
+      d.block_base = grp.box_blocks;
+      d.tiles = int32_t(x.tiles);
+      d.width = int32_t(x.width);
+      d.rows = int32_t(x.rows);
+      d.batch = int32_t(x.batch);
+      d.naxes = x.z - x.a + 1;
+      for (int32_t k = 0; k < d.naxes; ++k) {
+        int32_t const dim = x.a + k;
+        d.ext[k] = int32_t(b->extents_of(g)[dim]);
+        d.L[k] = int8_t(b->level_of(g)[dim]);
+        d.nlev[k] = int8_t(b->level_of(g)[dim] - b->lmin[dim]);
+      }
+      int64_t const n = (x.outer + x.batch - 1) / x.batch * x.tiles;
+      if (grp.box_blocks + n > 0x7fffff00LL)
+        return fail(PALIWA_ERR_UNSUPPORTED, "batch too large");
+      block_grid[i].insert(block_grid[i].end(), size_t(n), int32_t(grids[i].size()));
+      grids[i].push_back(d);
+      grp.box_blocks += n;
+    }
+  }
+  std::vector<size_t> grid_off(b->groups.size()), block_off(b->groups.size());
+  size_t bytes = 0;
+  for (size_t i = 0; i < b->groups.size(); ++i) {
+    grid_off[i] = bytes;
+    bytes = (bytes + grids[i].size() * sizeof(BoxGrid) + 255) & ~size_t(255);
+    block_off[i] = bytes;
+    bytes = (bytes + block_grid[i].size() * sizeof(int32_t) + 255) & ~size_t(255);
+  }
+  if (bytes == 0)
+    return PALIWA_OK;
+  std::vector<unsigned char> staging(bytes, 0);
+  for (size_t i = 0; i < b->groups.size(); ++i) {
+    std::memcpy(staging.data() + grid_off[i], grids[i].data(), grids[i].size() * sizeof(BoxGrid));
+    std::memcpy(staging.data() + block_off[i], block_grid[i].data(),
+                block_grid[i].size() * sizeof(int32_t));
+  }
+  if (hipMalloc(&b->box_dev, bytes) != hipSuccess ||
+      hipMemcpy(b->box_dev, staging.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    drop_box_tables(b);
+    return fail(PALIWA_ERR_HIP, "could not place the box descriptors");
+  }
+  b->box_dev_bytes = bytes;
+  auto *base = static_cast<unsigned char *>(b->box_dev);
+  for (size_t i = 0; i < b->groups.size(); ++i)
+    if (b->groups[i].box) {
+      b->groups[i].box_grids_dev = reinterpret_cast<BoxGrid *>(base + grid_off[i]);
+      b->groups[i].box_block_grid_dev = reinterpret_cast<int32_t *>(base + block_off[i]);
+    }
+  return PALIWA_OK;
+}
+
+// Points read per direction by the plan's current segments (boxes with the
+// empty part of their partial tiles).
+int64_t traffic(paliwa_batch const &b) {
+  int64_t total = 0;
+  for (int32_t g = 0; g < b.ngrids; ++g) {
+    int64_t e = 1;
+    for (int32_t d = 0; d < b.ndim; ++d)
+      e *= b.extents_of(g)[d];
+    for (Segment const &seg : segments_of(b, g))
+      total += seg.box ? seg.shape.outer * seg.shape.rows * seg.shape.tiles * seg.shape.width : e;
+  }
+  return total;
+}
+
 constexpr int kMaxLanes = 8;
+
+// Whether some phase has more than one group (then its groups run on the
+// internal streams).
+bool uses_lanes(paliwa_batch const &b) {
+  for (size_t i = 1; i < b.groups.size(); ++i)
+    if (b.groups[i].phase == b.groups[i - 1].phase)
+      return true;
+  return false;
+}
+// the box schedule a new plan takes when it pays: four boxes per CU, tile
+// rows of 512 bytes, 256 threads (float64; float32 boxes take 512 threads,
+// whose register rows then hold the whole box).  On the 1325-grid 4-D set
+// (|l|_1 in 13..16) this was the fastest of the tuner's 19 settings: 0.89 ms
+// per forward direction against 1.32 ms without boxes, 1.03 ms for 80 KiB
+// boxes and 1.26 ms or more for 160 KiB ones.
+constexpr int32_t kBoxRuleCapacity = 40 * 1024;
+constexpr int64_t kBoxRuleRowBytes = 512;
 
 // Lanes for the groups of each axis: largest group first onto the least
 // loaded of b->nlanes streams.  Work estimate in swept points: the group's
@@ -332,7 +598,7 @@ int setup_lanes(paliwa_batch *b) {
   bool concurrent = false;
   for (size_t i = 0; i < b->groups.size();) {
     size_t j = i;
-    while (j < b->groups.size() && b->groups[j].dim == b->groups[i].dim)
+    while (j < b->groups.size() && b->groups[j].phase == b->groups[i].phase)
       ++j;
     if (j - i > 1) {
       concurrent = true;
@@ -345,6 +611,8 @@ int setup_lanes(paliwa_batch *b) {
             e *= b->extents_of(g)[d];
           pts += e;
         }
+        for (auto const &[g, x] : b->groups[k].boxes)
+          pts += x.outer * x.rows * x.tiles * x.width;
         int64_t const launch = b->dtype == PALIWA_FLOAT64 ? 3750000 : 7500000;
         bool const long_lines = b->groups[k].lds_lines || b->groups[k].long_ladder;
         pts += long_lines ? b->long_weight * launch : launch;
@@ -380,6 +648,95 @@ int setup_lanes(paliwa_batch *b) {
   return PALIWA_OK;
 }
 
+// A group of single-axis sweeps: grids of one phase whose segment is axis d
+// with line period 2^L and stored extent m.
+paliwa_batch::Group single_group(paliwa_batch const &b, int32_t phase, int32_t d, int32_t L,
+                                 int32_t m, std::vector<int32_t> members) {
+  int32_t const dtype = b.dtype;
+  paliwa_batch::Group grp;
+  grp.phase = phase;
+  grp.dim = d;
+  grp.L = L;
+  grp.m = m;
+  grp.strided = d + 1 < b.ndim;
+  grp.members = std::move(members);
+  grp.fused = (int64_t(1) << grp.L) <= fused_max_period(dtype);
+  for (int32_t g : grp.members) {
+    SweepParams const q = make_params(b.data[g], b.ndim, b.extents_of(g), b.level_of(g), d);
+    grp.lines += q.A * q.B;
+  }
+  // long lines: the long-line kernel on strided axes, the LDS-line kernel on
+  // the contiguous one as far as it holds the line (as the single-grid
+  // schedule chooses), the long-line kernel beyond
+  int64_t const n = int64_t(1) << grp.L;
+  grp.long_ladder = !grp.fused && n <= long_ladder_max_period(dtype) &&
+                    (grp.strided || n > lds_line_max_period(dtype));
+  grp.lds_lines = !grp.fused && !grp.long_ladder && n <= lds_line_max_period(dtype);
+  if (grp.long_ladder) {
+    grp.threads = long_threads_for(dtype, grp.L, grp.lines);
+    int const G = int(n / (dtype == PALIWA_FLOAT64 ? 32 : 64));
+    if (G <= 256)
+      grp.widths = {256, 1024};
+  }
+  if (grp.fused && grp.strided) {
+    // the tile width of a strided float ladder follows the group's lines
+    int const K = fused_chunk(dtype, grp.L);
+    int const G = (1 << grp.L) / K;
+    int const eb = dtype == PALIWA_FLOAT64 ? 8 : 4;
+    grp.threads = ladder_threads_for(K, G, eb, grp.lines);
+    if (grp.threads != 0) {
+      grp.widths = {256, 512};
+      if (2 * (int64_t(K) * G + 1) * (512 / G) * eb <= kWideTileLds)
+        grp.widths.push_back(1024);
+    }
+  }
+  return grp;
+}
+
+// The plan's schedule from its box settings: the segments of every grid, the
+// groups of each phase (one box group, single-axis groups by axis, line
+// period and stored extent), their tables and streams.
+int build_structure(paliwa_batch *b) {
+  drop_graphs(b);
+  drop_box_tables(b);
+  b->groups.clear();
+  std::map<std::tuple<int32_t, int32_t, int32_t, int32_t>, std::vector<int32_t>> singles;
+  std::map<int32_t, std::vector<std::pair<int32_t, BoxShape>>> boxes;
+  for (int32_t g = 0; g < b->ngrids; ++g) {
+    std::vector<Segment> const segs = segments_of(*b, g);
+    for (size_t p = 0; p < segs.size(); ++p) {
+      Segment const &seg = segs[p];
+      int32_t const phase = b->box_capacity == 0 ? seg.shape.a : int32_t(p);
+      if (seg.box) {
+        boxes[phase].push_back({g, seg.shape});
+      } else {
+        int32_t const d = seg.shape.a;
+        singles[{phase, d, b->level_of(g)[d], int32_t(b->extents_of(g)[d])}].push_back(g);
+      }
+    }
+  }
+  for (int32_t phase = 0; phase < b->ndim; ++phase) {
+    auto const it = boxes.find(phase);
+    if (it != boxes.end()) {
+      paliwa_batch::Group grp;
+      grp.phase = phase;
+      grp.box = true;
+      grp.boxes = it->second;
+      b->groups.push_back(std::move(grp));
+    }
+    for (auto &[key, members] : singles)
+      if (std::get<0>(key) == phase)
+        b->groups.push_back(single_group(*b, phase, std::get<1>(key), std::get<2>(key),
+                                         std::get<3>(key), members));
+  }
+  int rc = build_tables(b);
+  if (rc == PALIWA_OK)
+    rc = build_box_tables(b);
+  if (rc == PALIWA_OK)
+    rc = setup_lanes(b);
+  return rc;
+}
+
 void release(paliwa_batch *b) {
   for (auto &kv : b->graphs)
     (void)hipGraphExecDestroy(kv.second);
@@ -393,6 +750,7 @@ void release(paliwa_batch *b) {
     (void)hipEventDestroy(b->fork);
   if (b->dev != nullptr)
     (void)hipFree(b->dev);
+  drop_box_tables(b);
   delete b;
 }
 
@@ -436,6 +794,7 @@ std::string schedule_key(paliwa_batch const &b, char const *wavelet, bool forwar
   h = fnv(fnv(h, b.dtype), b.ndim);
   for (auto const &grp : b.groups) {
     h = fnv(fnv(fnv(h, grp.dim), grp.L), grp.m);
+    h = fnv(fnv(h, grp.phase), grp.box ? int64_t(grp.boxes.size()) : -1);
     h = fnv(fnv(h, grp.lines), int64_t(grp.members.size()));
   }
   std::ostringstream k;
@@ -466,7 +825,8 @@ int write_cache(char const *path, TuneCache const &c) {
   out << "# paliwa batched-transform tuning cache (paliwa_batch_tune), one choice per line:\n"
          "#   width <dtype> <strided|contiguous> L=<log2 period> m=<stored extent> "
          "lines=2^<floor log2 of the group's lines> <wavelet> <direction> = <threads>\n"
-         "#   schedule <plan hash> <wavelet> <direction> = <streams> <long-line weight>\n";
+         "#   schedule <plan hash> <wavelet> <direction> = <streams> <long-line weight>\n"
+         "#   boxes <grid-set hash> <wavelet> <direction> = <box bytes> <tile width> <threads>\n";
   for (auto const &[k, v] : c)
     out << k << " = " << v << "\n";
   return out.good() ? PALIWA_OK : fail(PALIWA_ERR_IO, std::string("cannot write ") + path);
@@ -511,6 +871,27 @@ int time_direction(paliwa_batch *b, char const *wavelet, bool forward, hipStream
   return PALIWA_OK;
 }
 
+std::string boxes_key(paliwa_batch const &b, char const *wavelet, bool forward) {
+  uint64_t h = 1469598103934665603ull;
+  h = fnv(fnv(fnv(h, b.dtype), b.ndim), b.ngrids);
+  for (int32_t d = 0; d < b.ndim; ++d)
+    h = fnv(h, b.lmin[d]);
+  for (int32_t g = 0; g < b.ngrids; ++g)
+    for (int32_t d = 0; d < b.ndim; ++d)
+      h = fnv(fnv(h, b.level_of(g)[d]), b.extents_of(g)[d]);
+  std::ostringstream k;
+  k << "boxes " << std::hex << h << std::dec << " " << direction_tag(wavelet, forward);
+  return k.str();
+}
+
+bool valid_boxes(int32_t cap, int32_t width, int32_t threads) {
+  if (cap == 0)
+    return true;
+  bool const known = cap == kBoxCapacities[0] || cap == kBoxCapacities[1] || cap == kBoxCapacities[2];
+  return known && width >= 1 && width <= 4096 && (width & (width - 1)) == 0 &&
+         (threads == 256 || threads == 512);
+}
+
 int tune(paliwa_batch *b, char const *wavelet_name, bool forward, char const *cache_path,
          int32_t flags, hipStream_t s) {
   WaveletEntry const *w = find_wavelet(wavelet_name);
@@ -524,31 +905,56 @@ int tune(paliwa_batch *b, char const *wavelet_name, bool forward, char const *ca
   bool const force = flags & PALIWA_TUNE_FORCE;
   bool const measure = flags & PALIWA_TUNE_MEASURE;
   TuneCache cache = read_cache(cache_path);
-
-  // choices the cache answers, and what is left to search
-  std::vector<size_t> search;
-  std::vector<int32_t> rule_threads;
-  for (size_t i = 0; i < b->groups.size(); ++i) {
-    auto &grp = b->groups[i];
-    rule_threads.push_back(grp.threads);
-    if (grp.widths.empty())
-      continue;
-    auto const it = force ? cache.end() : cache.find(width_key(*b, grp, wname, forward));
-    int32_t cached = 0;
-    if (it != cache.end())
-      cached = std::atoi(it->second.c_str());
-    if (std::find(grp.widths.begin(), grp.widths.end(), cached) == grp.widths.end())
-      search.push_back(i);
-  }
+  int32_t const rule_box[3] = {b->box_capacity, b->box_width, b->box_threads};
   int32_t const rule_lanes = b->nlanes, rule_weight = b->long_weight;
-  std::string const skey = schedule_key(*b, wname, forward);
-  bool search_schedule = force || cache.find(skey) == cache.end();
-  bool const concurrent = !b->lanes.empty();
-  if (!concurrent)
-    search_schedule = false;
+  auto set_boxes = [&](int32_t cap, int32_t width, int32_t threads) {
+    b->box_capacity = cap;
+    b->box_width = cap > 0 ? width : 0;
+    b->box_threads = cap > 0 ? threads : 256;
+    return build_structure(b);
+  };
+
+  // the box schedule the cache holds (applied first: the groups' keys depend
+  // on it)
+  std::string const bkey = boxes_key(*b, wname, forward);
+  int32_t cached_box[3] = {0, 0, 0};
+  bool search_boxes = true;
+  if (flags & PALIWA_TUNE_KEEP_BOXES) {
+    cached_box[0] = rule_box[0];
+    cached_box[1] = rule_box[1];
+    cached_box[2] = rule_box[2];
+    search_boxes = false;
+  } else if (!force) {
+    auto const it = cache.find(bkey);
+    std::istringstream v(it == cache.end() ? std::string() : it->second);
+    if (v >> cached_box[0] >> cached_box[1] >> cached_box[2] &&
+        valid_boxes(cached_box[0], cached_box[1], cached_box[2]))
+      search_boxes = false;
+  }
+  int rc = PALIWA_OK;
+  if (!search_boxes)
+    rc = set_boxes(cached_box[0], cached_box[1], cached_box[2]);
+
+  // choices of the groups the cache answers, and what is left to search
+  auto group_search = [&]() {
+    std::vector<size_t> search;
+    for (size_t i = 0; i < b->groups.size(); ++i) {
+      auto const &grp = b->groups[i];
+      if (grp.widths.empty())
+        continue;
+      auto const it = force ? cache.end() : cache.find(width_key(*b, grp, wname, forward));
+      int32_t const cached = it != cache.end() ? std::atoi(it->second.c_str()) : 0;
+      if (std::find(grp.widths.begin(), grp.widths.end(), cached) == grp.widths.end())
+        search.push_back(i);
+    }
+    return search;
+  };
+  std::vector<size_t> search = group_search();
+  std::string skey = schedule_key(*b, wname, forward);
+  bool search_schedule = uses_lanes(*b) && (force || cache.find(skey) == cache.end());
   b->evaluations = 0;
   b->rule_ms = b->tuned_ms = -1.0;
-  bool const timing = measure || !search.empty() || search_schedule;
+  bool const timing = measure || search_boxes || !search.empty() || search_schedule;
 
   void *backup = nullptr;
   size_t const es = b->dtype == PALIWA_FLOAT64 ? 8 : 4;
@@ -576,27 +982,68 @@ int tune(paliwa_batch *b, char const *wavelet_name, bool forward, char const *ca
     return hipStreamSynchronize(s) == hipSuccess ? PALIWA_OK
                                                  : fail(PALIWA_ERR_HIP, "copy failed");
   };
-  int rc = PALIWA_OK;
   // candidates are timed as direct launches (no graph captured per candidate)
   bool const graphs = b->use_graphs;
   b->use_graphs = false;
-  if (timing) {
+  if (rc == PALIWA_OK && timing) {
     if (hipMalloc(&backup, total) != hipSuccess) {
-      b->use_graphs = graphs;
+      backup = nullptr;
       (void)hipGetLastError();
-      return fail(PALIWA_ERR_HIP, "no device memory for the tuning copy of the grids");
+      rc = fail(PALIWA_ERR_HIP, "no device memory for the tuning copy of the grids");
+    } else {
+      rc = copy_all(true);
     }
-    rc = copy_all(true);
   }
   constexpr int kReps = 15;
   auto evaluate = [&](double *ms) {
     ++b->evaluations;
     return time_direction(b, wname, forward, s, kReps, ms);
   };
-  if (rc == PALIWA_OK && timing)
-    rc = evaluate(&b->rule_ms);
+  // the plan as the rules made it
+  if (rc == PALIWA_OK && timing) {
+    rc = set_boxes(rule_box[0], rule_box[1], rule_box[2]);
+    if (rc == PALIWA_OK)
+      rc = evaluate(&b->rule_ms);
+  }
+  double best = b->rule_ms;
 
-  // apply the cached choices
+  // the box schedule: none, or capacity x tile row (128 / 256 / 512 bytes) x
+  // threads, the whole direction timed for each
+  if (rc == PALIWA_OK && search_boxes) {
+    int32_t keep[3] = {rule_box[0], rule_box[1], rule_box[2]};
+    std::vector<std::array<int32_t, 3>> candidates = {{0, 0, 256}};
+    for (int32_t cap : kBoxCapacities)
+      for (int32_t row : {128, 256, 512})
+        for (int32_t threads : {256, 512})
+          candidates.push_back({cap, int32_t(row / int32_t(es)), threads});
+    for (auto const &c : candidates) {
+      if (rc != PALIWA_OK)
+        break;
+      if (c[0] == rule_box[0] && (c[0] == 0 || (c[1] == rule_box[1] && c[2] == rule_box[2])))
+        continue;
+      double ms = 0.0;
+      if ((rc = set_boxes(c[0], c[1], c[2])) == PALIWA_OK && (rc = evaluate(&ms)) == PALIWA_OK &&
+          ms < 0.99 * best) {
+        best = ms;
+        keep[0] = c[0];
+        keep[1] = c[1];
+        keep[2] = c[2];
+      }
+    }
+    cached_box[0] = keep[0];
+    cached_box[1] = keep[1];
+    cached_box[2] = keep[2];
+  }
+  if (rc == PALIWA_OK)
+    rc = set_boxes(cached_box[0], cached_box[1], cached_box[2]);
+  if (rc == PALIWA_OK && search_boxes) { // the groups of the chosen schedule
+    search = group_search();
+    skey = schedule_key(*b, wname, forward);
+    search_schedule = uses_lanes(*b) && (force || cache.find(skey) == cache.end());
+  }
+  bool const concurrent = uses_lanes(*b);
+
+  // apply the cached choices of the groups
   if (rc == PALIWA_OK) {
     for (auto &grp : b->groups)
       if (!grp.widths.empty() && !force) {
@@ -619,8 +1066,7 @@ int tune(paliwa_batch *b, char const *wavelet_name, bool forward, char const *ca
     if (rc == PALIWA_OK)
       rc = setup_lanes(b);
   }
-  double best = b->rule_ms;
-  if (rc == PALIWA_OK && timing && (!search.empty() || search_schedule || measure))
+  if (rc == PALIWA_OK && timing)
     rc = evaluate(&best);
   // coordinate descent: one choice at a time, kept when at least 1% faster
   std::sort(search.begin(), search.end(), [&](size_t x, size_t y) {
@@ -681,16 +1127,16 @@ int tune(paliwa_batch *b, char const *wavelet_name, bool forward, char const *ca
       rc = restore;
   }
   if (rc != PALIWA_OK) { // leave the plan as the rules made it
-    for (size_t i = 0; i < b->groups.size(); ++i)
-      b->groups[i].threads = rule_threads[i];
+    std::string const msg = last_error();
     b->nlanes = rule_lanes;
     b->long_weight = rule_weight;
-    std::string const msg = last_error();
-    if (build_tables(b) == PALIWA_OK)
-      (void)setup_lanes(b);
+    (void)set_boxes(rule_box[0], rule_box[1], rule_box[2]);
     return fail(rc, msg);
   }
   if (cache_path != nullptr) {
+    if (!(flags & PALIWA_TUNE_KEEP_BOXES))
+      cache[bkey] = std::to_string(b->box_capacity) + " " + std::to_string(b->box_width) + " " +
+                    std::to_string(b->box_threads);
     for (auto const &grp : b->groups)
       if (!grp.widths.empty())
         cache[width_key(*b, grp, wname, forward)] = std::to_string(grp.threads);
@@ -742,55 +1188,17 @@ int paliwa_batch_create(paliwa_batch **plan, int32_t dtype, int32_t ndim,
     return fail(PALIWA_ERR_HIP, "no HIP device");
   }
 
-  // Group the sweeps: per axis, grids with equal (line period, stored extent).
-  for (int32_t d = 0; d < ndim; ++d) {
-    std::map<std::pair<int32_t, int32_t>, std::vector<int32_t>> by_shape;
-    for (int32_t g = 0; g < ngrids; ++g)
-      if (b->level_of(g)[d] > b->lmin[d])
-        by_shape[{b->level_of(g)[d], int32_t(b->extents_of(g)[d])}].push_back(g);
-    for (auto &[shape, members] : by_shape) {
-      paliwa_batch::Group grp;
-      grp.dim = d;
-      grp.L = shape.first;
-      grp.m = shape.second;
-      grp.strided = d + 1 < ndim;
-      grp.members = members;
-      grp.fused = (int64_t(1) << grp.L) <= fused_max_period(dtype);
-      for (int32_t g : members) {
-        SweepParams const q = make_params(b->data[g], ndim, b->extents_of(g), b->level_of(g), d);
-        grp.lines += q.A * q.B;
-      }
-      // long lines: the long-line kernel on strided axes, the LDS-line kernel
-      // on the contiguous one as far as it holds the line (as the single-grid
-      // schedule chooses), the long-line kernel beyond
-      int64_t const n = int64_t(1) << grp.L;
-      grp.long_ladder = !grp.fused && n <= long_ladder_max_period(dtype) &&
-                        (grp.strided || n > lds_line_max_period(dtype));
-      grp.lds_lines = !grp.fused && !grp.long_ladder && n <= lds_line_max_period(dtype);
-      if (grp.long_ladder) {
-        grp.threads = long_threads_for(dtype, grp.L, grp.lines);
-        int const G = int(n / (dtype == PALIWA_FLOAT64 ? 32 : 64));
-        if (G <= 256)
-          grp.widths = {256, 1024};
-      }
-      if (grp.fused && grp.strided) {
-        // the tile width of a strided float ladder follows the group's lines
-        int const K = fused_chunk(dtype, grp.L);
-        int const G = (1 << grp.L) / K;
-        int const eb = dtype == PALIWA_FLOAT64 ? 8 : 4;
-        grp.threads = ladder_threads_for(K, G, eb, grp.lines);
-        if (grp.threads != 0) {
-          grp.widths = {256, 512};
-          if (2 * (int64_t(K) * G + 1) * (512 / G) * eb <= kWideTileLds)
-            grp.widths.push_back(1024);
-        }
-      }
-      b->groups.push_back(std::move(grp));
-    }
+  // the box schedule where it reads at most three quarters of what the
+  // per-axis sweeps read
+  int64_t const per_axis = traffic(*b);
+  b->box_capacity = kBoxRuleCapacity;
+  b->box_width = int32_t(kBoxRuleRowBytes / (dtype == PALIWA_FLOAT64 ? 8 : 4));
+  b->box_threads = dtype == PALIWA_FLOAT64 ? 256 : 512;
+  if (4 * traffic(*b) > 3 * per_axis) {
+    b->box_capacity = 0;
+    b->box_width = 0;
   }
-  int rc = build_tables(b);
-  if (rc == PALIWA_OK)
-    rc = setup_lanes(b);
+  int rc = build_structure(b);
   if (rc != PALIWA_OK) {
     std::string const msg = last_error();
     release(b);
@@ -818,13 +1226,56 @@ int paliwa_batch_set_graphs(paliwa_batch *plan, int32_t enable) {
   return PALIWA_OK;
 }
 
+int paliwa_batch_set_boxes(paliwa_batch *plan, int32_t capacity_bytes, int32_t width,
+                           int32_t threads) {
+  if (plan == nullptr)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "plan is NULL");
+  bool const known = capacity_bytes == 0 || capacity_bytes == kBoxCapacities[0] ||
+                     capacity_bytes == kBoxCapacities[1] || capacity_bytes == kBoxCapacities[2];
+  if (!known)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "box capacity must be 0, 40960, 81920 or 163840");
+  if (capacity_bytes > 0 &&
+      (width < 1 || width > 4096 || (width & (width - 1)) != 0 || (threads != 256 && threads != 512)))
+    return fail(PALIWA_ERR_INVALID_ARGUMENT,
+                "box width must be a power of two in [1, 4096] and threads 256 or 512");
+  int32_t const keep[3] = {plan->box_capacity, plan->box_width, plan->box_threads};
+  plan->box_capacity = capacity_bytes;
+  plan->box_width = capacity_bytes > 0 ? width : 0;
+  plan->box_threads = capacity_bytes > 0 ? threads : 256;
+  int rc = build_structure(plan);
+  if (rc != PALIWA_OK) {
+    std::string const msg = last_error();
+    plan->box_capacity = keep[0];
+    plan->box_width = keep[1];
+    plan->box_threads = keep[2];
+    (void)build_structure(plan);
+    return fail(rc, msg);
+  }
+  last_error().clear();
+  return PALIWA_OK;
+}
+
+int paliwa_batch_get_boxes(paliwa_batch const *plan, int32_t *capacity_bytes, int32_t *width,
+                           int32_t *threads) {
+  if (plan == nullptr)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "plan is NULL");
+  if (capacity_bytes != nullptr)
+    *capacity_bytes = plan->box_capacity;
+  if (width != nullptr)
+    *width = plan->box_width;
+  if (threads != nullptr)
+    *threads = plan->box_threads;
+  return PALIWA_OK;
+}
+
 int paliwa_batch_info(paliwa_batch const *plan, int32_t *launches,
                       int64_t *points, int64_t *descriptor_bytes) {
   if (plan == nullptr)
     return fail(PALIWA_ERR_INVALID_ARGUMENT, "plan is NULL");
   int32_t n = 0;
   for (auto const &grp : plan->groups)
-    n += grp.fused || grp.long_ladder || grp.lds_lines ? 1 : int32_t(grp.members.size());
+    n += grp.box || grp.fused || grp.long_ladder || grp.lds_lines ? 1
+                                                                   : int32_t(grp.members.size());
   int64_t total = 0;
   for (int32_t g = 0; g < plan->ngrids; ++g) {
     int64_t e = 1;
@@ -837,7 +1288,7 @@ int paliwa_batch_info(paliwa_batch const *plan, int32_t *launches,
   if (points != nullptr)
     *points = total;
   if (descriptor_bytes != nullptr)
-    *descriptor_bytes = int64_t(plan->dev_bytes);
+    *descriptor_bytes = int64_t(plan->dev_bytes + plan->box_dev_bytes);
   return PALIWA_OK;
 }
 
@@ -867,7 +1318,7 @@ int paliwa_batch_tune_info(paliwa_batch const *plan, double *rule_ms, double *tu
   if (evaluations != nullptr)
     *evaluations = plan->evaluations;
   if (streams != nullptr)
-    *streams = plan->lanes.empty() ? 0 : plan->nlanes;
+    *streams = uses_lanes(*plan) ? plan->nlanes : 0;
   return PALIWA_OK;
 }
 

@@ -206,4 +206,48 @@ hipError_t launch_long_ladder(int dtype, SweepParams const &p, hipStream_t strea
 hipError_t launch_long_ladder_batched(int dtype, SweepParams const &p, bool strided,
                                       ManyGrids grids, int64_t tiles, hipStream_t stream);
 
+// Boxes (box.hip): consecutive axes a..b of a grid swept together in LDS.  A
+// box is the sub-array of one index of the axes before a (the outer index),
+// the full extents of axes a..b, and one tile of `width` consecutive indices
+// of the flattened axes after b (the inner index); every axis of the box runs
+// all its levels there, in axis order, so the box costs one HBM read and one
+// write for its axes together.
+constexpr int kBoxMaxAxes = 8;
+struct BoxGrid {
+  void *data;
+  int64_t outer_stride; // points per outer index: product of the extents a..D-1
+  int64_t inner;        // points per index of axis b: product of the extents after b
+  int64_t outer;        // outer indices: product of the extents before a
+  int64_t block_base;   // index of the grid's first workgroup in the launch
+  int32_t tiles;        // tiles of the inner range per outer index
+  int32_t width;        // inner points per tile: all of them, or a power of two
+  int32_t rows;         // product of the box axes' extents
+  int32_t batch;        // consecutive outer indices per workgroup (whole-inner tiles)
+  int32_t naxes;
+  int32_t ext[kBoxMaxAxes]; // stored extents of the box axes
+  int8_t L[kBoxMaxAxes];    // log2 line periods
+  int8_t nlev[kBoxMaxAxes]; // levels swept per axis (level - minimum level)
+};
+// One launch: `blocks` boxes of the grids in `grids`, box -> grid through
+// `block_grid`; every axis runs the named-transform levels of its direction
+// with the passes of `filters`.
+struct BoxLaunch {
+  BoxGrid const *grids;
+  int32_t const *block_grid;
+  int64_t blocks;
+  int32_t forward;
+  int32_t npass;     // 1 or 2
+  int32_t offset[2]; // 32-bit: read with scalar loads
+  double coef[2][3];
+};
+// LDS of a box workgroup (bytes): 40, 80 or 160 KiB, four, two or one per CU;
+// the box's axis table takes kBoxTableBytes of it, its points the rest.
+constexpr int kBoxCapacities[3] = {40 * 1024, 80 * 1024, 160 * 1024};
+constexpr int kBoxTableBytes = 128;
+constexpr int64_t box_points(int capacity_bytes, int elem_bytes) {
+  return (capacity_bytes - kBoxTableBytes) / elem_bytes;
+}
+hipError_t launch_box_batched(int dtype, BoxLaunch const &launch, int capacity_bytes,
+                              int threads, hipStream_t stream);
+
 } // namespace paliwa_hip

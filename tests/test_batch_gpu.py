@@ -38,7 +38,8 @@ def make_grids(levels, image, dtype, seed):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("image", [True, False])
-def test_batch_matches_oracle(pw, dtype, image):
+@pytest.mark.parametrize("boxes", [True, False])
+def test_batch_matches_oracle(pw, dtype, image, boxes):
     levels = pw.combination_levels(3, (6, 7), 1)
     lmin, lmax = [1, 0, 1], [5, 5, 5]
     arrays = make_grids(levels, image, dtype, seed=21)
@@ -46,6 +47,8 @@ def test_batch_matches_oracle(pw, dtype, image):
         for graphs in (True, False):
             grids = [torch.from_numpy(a).cuda() for a in arrays]
             batch = pw.CombinationBatch(grids, levels, lmin, lmax)
+            if not boxes:
+                batch.set_boxes(0)
             batch.use_graphs(graphs)
             batch.hierarchize(name)
             torch.cuda.synchronize()
@@ -74,6 +77,7 @@ def test_batch_six_dimensions(pw):
         state = make_grids(levels, image, np.float64, seed=23)
         grids = [torch.from_numpy(a).cuda() for a in state]
         batch = pw.CombinationBatch(grids, levels, lmin, lmax)
+        batch.set_boxes(0)  # six axis phases of groups
         # each wavelet starts from the previous one's round trip (not bit-exact)
         for name in ("hat", "biorthogonal"):
             batch.hierarchize(name)
@@ -100,6 +104,7 @@ def test_batch_wide_tiles(pw, dtype):
     for name in ("hat", "biorthogonal"):
         grids = [torch.from_numpy(a).cuda() for a in arrays]
         batch = pw.CombinationBatch(grids, levels, lmin, lmax)
+        batch.set_boxes(0)  # the per-axis groups' wide tiles
         batch.hierarchize(name)
         torch.cuda.synchronize()
         wants = []
@@ -128,6 +133,7 @@ def test_batch_groups_over_many_lines(pw, dtype, levels):
     arrays = make_grids(levels, True, dtype, seed=25)
     grids = [torch.from_numpy(a).cuda() for a in arrays]
     batch = pw.CombinationBatch(grids, levels, lmin, lmax)
+    batch.set_boxes(0)  # the per-axis groups' tile tables
     batch.hierarchize("biorthogonal")
     torch.cuda.synchronize()
     wants = []
@@ -157,6 +163,7 @@ def test_batch_long_lines(pw, dtype):
         for graphs in (True, False):
             grids = [torch.from_numpy(a).cuda() for a in arrays]
             batch = pw.CombinationBatch(grids, levels, lmin, lmax)
+            batch.set_boxes(0)  # the per-axis long-line groups
             batch.use_graphs(graphs)
             batch.hierarchize(name)
             torch.cuda.synchronize()
@@ -215,12 +222,22 @@ def test_combination_technique_set_1325_grids(pw):
     sample = list(range(0, 1325, 97))
     initial = {i: grids[i].cpu().numpy() for i in sample}
     batch = pw.CombinationBatch(grids, levels, lmin, lmax)
+    assert batch.boxes() == (40960, 64, 256)  # the rule's box schedule
     batch.hierarchize("hat")
     for g, l in zip(singles, levels):
         pw.hierarchize(g, l, lmin, lmax, "hat")
     torch.cuda.synchronize()
     for g, s in zip(grids, singles):
         assert torch.equal(g, s)
+    # the per-axis groups on a copy: the same bits
+    copies = [torch.from_numpy(initial[i]).cuda() for i in sample]
+    per_axis = pw.CombinationBatch(copies, [levels[i] for i in sample], lmin, lmax)
+    per_axis.set_boxes(0)
+    per_axis.hierarchize("hat")
+    torch.cuda.synchronize()
+    for c, i in zip(copies, sample):
+        assert torch.equal(c, grids[i])
+    per_axis.close()
     # a sample against the CPU oracle
     for i in sample:
         want = initial[i].copy()

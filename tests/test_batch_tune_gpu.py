@@ -65,7 +65,7 @@ def test_tuned_plan_is_bit_identical(pw, dtype, tmp_path):
         for g, h in zip(tuned, plain):
             assert np.array_equal(g.cpu().numpy(), h.cpu().numpy())
         text = cache.read_text()
-        assert f"{name} forward" in text and "width " in text and "schedule " in text
+        assert f"{name} forward" in text and "boxes " in text
         # a second plan over the same shapes takes every choice from the cache
         again = pw.CombinationBatch([torch.from_numpy(a).cuda() for a in arrays], levels, lmin,
                                     lmax)
@@ -75,6 +75,39 @@ def test_tuned_plan_is_bit_identical(pw, dtype, tmp_path):
         plan.close()
         rule.close()
         again.close()
+
+
+def test_group_widths_tuned_under_kept_boxes(pw, tmp_path):
+    """With the per-axis groups kept (no box schedule), the tuner chooses the
+    strided groups' tile widths and the stream schedule: bit-identical
+    results, and a cache that answers a second plan."""
+    levels = [l for l in pw.combination_levels(3, (11, 12), 1) if max(l) <= 10]
+    lmin, lmax = [0, 0, 0], [10, 10, 10]
+    arrays = grids_for(levels, np.float64, seed=6)
+    cache = tmp_path / "tune.txt"
+    tuned = [torch.from_numpy(a).cuda() for a in arrays]
+    plan = pw.CombinationBatch(tuned, levels, lmin, lmax)
+    plan.set_boxes(0)
+    report = plan.tune("biorthogonal", forward=False, cache=cache, force=True, measure=True,
+                       keep_boxes=True)
+    assert report["evaluations"] >= 3 and plan.boxes()[0] == 0
+    keys = [line.split()[0] for line in cache.read_text().splitlines()
+            if line and not line.startswith("#")]
+    assert "width" in keys and "schedule" in keys and "boxes" not in keys
+    plain = [torch.from_numpy(a).cuda() for a in arrays]
+    rule = pw.CombinationBatch(plain, levels, lmin, lmax)
+    for p in (plan, rule):
+        p.hierarchize("biorthogonal")
+        p.dehierarchize("biorthogonal")
+    torch.cuda.synchronize()
+    for g, h in zip(tuned, plain):
+        assert np.array_equal(g.cpu().numpy(), h.cpu().numpy())
+    again = pw.CombinationBatch([torch.from_numpy(a).cuda() for a in arrays], levels, lmin, lmax)
+    again.set_boxes(0)
+    second = again.tune("biorthogonal", forward=False, cache=cache, keep_boxes=True)
+    assert second["evaluations"] == 0 and second["streams"] == report["streams"]
+    for p in (plan, rule, again):
+        p.close()
 
 
 def test_float32_tuning_is_refused(pw):
