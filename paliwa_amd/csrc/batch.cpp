@@ -207,11 +207,16 @@ int launch_sequence(paliwa_batch &b, WaveletEntry const &w, bool forward,
       for (size_t t = 0; t < b.lanes.size() && rc == PALIWA_OK; ++t)
         if (used[t])
           rc = hip(hipStreamWaitEvent(b.lanes[t], b.fork, 0));
-      // long-line groups first: few workgroups each, latency-bound, so they
-      // run beside the large groups instead of after them
-      for (int pass = 0; pass < 2 && rc == PALIWA_OK; ++pass)
+      // the box group first (the phase's largest launch, which should not
+      // queue behind others), then long-line groups (few workgroups each,
+      // latency-bound, so they run beside the large groups instead of after
+      // them), then the rest
+      auto order = [&](paliwa_batch::Group const &g) {
+        return g.box ? 0 : (g.lds_lines || g.long_ladder) ? 1 : 2;
+      };
+      for (int pass = 0; pass < 3 && rc == PALIWA_OK; ++pass)
         for (size_t k = i; k < j && rc == PALIWA_OK; ++k)
-          if ((b.groups[k].lds_lines || b.groups[k].long_ladder) == (pass == 0))
+          if (order(b.groups[k]) == pass)
             rc = launch_group(b, b.groups[k], filters, nfilters, forward,
                               b.lanes[b.lane_of[k]]);
       for (size_t t = 0; t < b.lanes.size() && rc == PALIWA_OK; ++t)
@@ -982,9 +987,8 @@ int tune(paliwa_batch *b, char const *wavelet_name, bool forward, char const *ca
     return hipStreamSynchronize(s) == hipSuccess ? PALIWA_OK
                                                  : fail(PALIWA_ERR_HIP, "copy failed");
   };
-  // candidates are timed as direct launches (no graph captured per candidate)
-  bool const graphs = b->use_graphs;
-  b->use_graphs = false;
+  // candidates are timed as the plan executes them: as graph replays (each
+  // candidate's graph captured in its warm-up) unless graphs are switched off
   if (rc == PALIWA_OK && timing) {
     if (hipMalloc(&backup, total) != hipSuccess) {
       backup = nullptr;
@@ -1119,7 +1123,6 @@ int tune(paliwa_batch *b, char const *wavelet_name, bool forward, char const *ca
   }
   if (timing)
     b->tuned_ms = best;
-  b->use_graphs = graphs;
   if (backup != nullptr) {
     int const restore = copy_all(false);
     (void)hipFree(backup);
