@@ -297,8 +297,9 @@ def ct_benchmark(torch, pw, fields, reps=5):
         return e0.elapsed_time(e1) / n
 
     rule_ms = timed(lambda: batch.hierarchize("hat"), reps)
-    # the plan's autotuner: tile widths and stream schedule timed on these
-    # grids (restored afterwards), no cache so every run tunes afresh
+    # the plan's autotuner: box schedule, tile widths and stream schedule
+    # timed on these grids (restored afterwards), no cache so every run tunes
+    # afresh
     tune = batch.tune("hat", forward=True, force=True, measure=True)
     batched_ms = timed(lambda: batch.hierarchize("hat"), reps)
 
@@ -331,6 +332,7 @@ def ct_benchmark(torch, pw, fields, reps=5):
         s = b.abs().max().item()
         max_rel = max(max_rel, d / s if s > 0 else d)
     info = batch.info()
+    boxes = batch.boxes()
     batch.close()
     bytes_fwd = 2 * 4 * points * 8
     combine = combine_benchmark(torch, levels, out_batched, timed, reps)
@@ -348,7 +350,13 @@ def ct_benchmark(torch, pw, fields, reps=5):
                   "evaluations": tune["evaluations"], "streams": tune["streams"]},
         "per_grid_ms": round(per_grid_ms, 4),
         "speedup": round(per_grid_ms / batched_ms, 2),
-        "batched_GBps": round(bytes_fwd / (batched_ms * 1e-3) / 1e9, 1),
+        # the metric's bytes: one read and one write of every point per axis
+        "algorithmic_GBps": round(bytes_fwd / (batched_ms * 1e-3) / 1e9, 1),
+        # what the plan moves: one read and write per segment (box or axis)
+        "plan_bytes": 2 * info["points_read_per_direction"] * 8,
+        "plan_GBps": round(2 * info["points_read_per_direction"] * 8
+                           / (batched_ms * 1e-3) / 1e9, 1),
+        "boxes": list(boxes),
         "batched_launches": info["launches_per_direction"],
         "per_grid_launches": 4 * len(levels),
         "max_rel_diff_batched_vs_per_grid": max_rel,

@@ -283,6 +283,12 @@ int paliwa_batch_set_boxes(paliwa_batch *plan, int32_t capacity_bytes,
 int paliwa_batch_get_boxes(const paliwa_batch *plan, int32_t *capacity_bytes,
                            int32_t *width, int32_t *threads);
 
+/* Points one direction of the plan reads from device memory (and writes
+ * back): each grid once per segment of its schedule -- once per axis with
+ * levels without boxes, once per box segment with them (with the empty part
+ * of partial tiles) -- the model of the plan's memory traffic. */
+int paliwa_batch_traffic(const paliwa_batch *plan, int64_t *points_read);
+
 /* Kernel launches per direction, total stored points, descriptor bytes. */
 int paliwa_batch_info(const paliwa_batch *plan, int32_t *launches,
                       int64_t *points, int64_t *descriptor_bytes);

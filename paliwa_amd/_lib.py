@@ -122,6 +122,7 @@ SIGNATURES = {
     "paliwa_batch_set_graphs": (ctypes.c_int, [_vp, _i32]),
     "paliwa_batch_info": (ctypes.c_int, [_vp, _i32p, _i64p, _i64p]),
     "paliwa_batch_set_boxes": (ctypes.c_int, [_vp, _i32, _i32, _i32]),
+    "paliwa_batch_traffic": (ctypes.c_int, [_vp, _i64p]),
     "paliwa_batch_get_boxes": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p]),
     "paliwa_batch_destroy": (ctypes.c_int, [_vp]),
     "paliwa_batch_tune": (ctypes.c_int, [_vp, ctypes.c_char_p, _i32, ctypes.c_char_p, _i32, _vp]),

@@ -100,8 +100,10 @@ class CombinationBatch:
         nbytes = ctypes.c_int64(0)
         check(_lib.lib.paliwa_batch_info(self._plan, ctypes.byref(launches),
                                          ctypes.byref(points), ctypes.byref(nbytes)))
+        read = ctypes.c_int64(0)
+        check(_lib.lib.paliwa_batch_traffic(self._plan, ctypes.byref(read)))
         return {"launches_per_direction": launches.value, "points": points.value,
-                "descriptor_bytes": nbytes.value}
+                "descriptor_bytes": nbytes.value, "points_read_per_direction": read.value}
 
     def tune(self, wavelet_name: str = "hat", forward: bool = True, cache=None,
              force: bool = False, measure: bool = False, keep_boxes: bool = False,

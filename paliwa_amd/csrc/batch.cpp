@@ -1271,6 +1271,13 @@ int paliwa_batch_get_boxes(paliwa_batch const *plan, int32_t *capacity_bytes, in
   return PALIWA_OK;
 }
 
+int paliwa_batch_traffic(paliwa_batch const *plan, int64_t *points_read) {
+  if (plan == nullptr || points_read == nullptr)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "plan and points_read are required");
+  *points_read = traffic(*plan);
+  return PALIWA_OK;
+}
+
 int paliwa_batch_info(paliwa_batch const *plan, int32_t *launches,
                       int64_t *points, int64_t *descriptor_bytes) {
   if (plan == nullptr)

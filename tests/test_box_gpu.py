@@ -150,3 +150,18 @@ def test_box_settings_are_checked(pw):
     batch.set_boxes(0)
     assert batch.boxes() == (0, 0, 256)
     batch.close()
+
+
+def test_box_schedule_traffic(pw):
+    """The plan's traffic model: 17^4 grids (levels 4, images) take axis 0
+    alone and axes 1..3 as one whole box in 40 KiB, so a direction reads
+    every point twice instead of four times; the groups read it four times."""
+    grids = [torch.zeros((17,) * 4, dtype=torch.float64, device="cuda") for _ in range(3)]
+    batch = pw.CombinationBatch(grids, [(4,) * 4] * 3, [0] * 4, [4] * 4)
+    batch.set_boxes(40960, 64, 256)
+    points = 3 * 17 ** 4
+    assert batch.info()["points_read_per_direction"] == 2 * points
+    assert batch.info()["launches_per_direction"] == 2
+    batch.set_boxes(0)
+    assert batch.info()["points_read_per_direction"] == 4 * points
+    batch.close()
