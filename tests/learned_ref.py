@@ -27,15 +27,41 @@ def bf16(x):
     return (r.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
-def gelu(z):
+# The kernels' GELU: z * g(t), t = clip(z, -c, c), g = clip(0.5 + t P(t^2), 0, 1)
+# (paliwa_amd/csrc/learned.hip), a fit to the tanh form; test_learned_ref.py
+# pins it against that form and its derivative against differences.
+GELU_CLAMP = np.float32(3.4513)
+GELU_P = np.array([3.93313942e-01, -5.90118521e-02, 6.50023659e-03, -3.92342173e-04,
+                   9.70034432e-06], dtype=np.float32).astype(np.float64)
+GELU_CLAMP64 = float(GELU_CLAMP)
+
+
+def gelu_tanh(z):
+    """The tanh form the kernels' GELU approximates."""
     return 0.5 * z * (1.0 + np.tanh(0.7978845608028654 * (z + 0.044715 * z ** 3)))
 
 
+def _gelu_parts(z):
+    z = np.asarray(z, dtype=np.float64)
+    t = np.clip(z, -GELU_CLAMP64, GELU_CLAMP64)
+    u = t * t
+    p = np.polyval(GELU_P[::-1], u)
+    raw = 0.5 + t * p
+    return z, t, u, p, raw
+
+
+def gelu(z):
+    z, t, u, p, raw = _gelu_parts(z)
+    return z * np.clip(raw, 0.0, 1.0)
+
+
 def gelu_grad(z):
-    a = 0.7978845608028654 * (z + 0.044715 * z ** 3)
-    t = np.tanh(a)
-    da = 0.7978845608028654 * (1.0 + 3.0 * 0.044715 * z * z)
-    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * da
+    z, t, u, p, raw = _gelu_parts(z)
+    g = np.clip(raw, 0.0, 1.0)
+    dp = np.polyval((GELU_P[1:] * np.arange(1, 5))[::-1], u)
+    dg = p + 2.0 * u * dp
+    inside = (np.abs(z) < GELU_CLAMP64) & (raw > 0.0) & (raw < 1.0)
+    return np.where(inside, g + z * dg, g)
 
 
 def unpack(w):

@@ -32,6 +32,27 @@ def mlp64(u, w):
     return h2 @ w3 + b3
 
 
+def test_gelu_is_the_tanh_form_within_its_fit():
+    """The kernels' transcendental-free GELU stays within 1.3e-3 of the tanh
+    form everywhere (a dense grid out to |z| = 60), is exactly z and 0 beyond
+    the clamp, and is odd-symmetric about z/2 (gelu(z) - gelu(-z) = z)."""
+    z = np.concatenate([np.linspace(-12, 12, 240001), np.linspace(-60, 60, 12001)])
+    err = np.abs(lr.gelu(z) - lr.gelu_tanh(z))
+    assert err.max() <= 1.3e-3, err.max()
+    big = np.array([3.5, 5.0, 40.0])
+    assert np.array_equal(lr.gelu(big), big) and np.array_equal(lr.gelu(-big), 0 * big)
+    assert np.allclose(lr.gelu(z) - lr.gelu(-z), z, rtol=0, atol=1e-6)
+
+
+def test_gelu_grad_matches_differences():
+    rng = np.random.default_rng(5)
+    z = np.concatenate([rng.uniform(-5, 5, 4000), [0.0, 1e-3, -1e-3, 3.0, -3.0]])
+    z = z[np.abs(np.abs(z) - lr.GELU_CLAMP64) > 1e-4]  # away from the clamp's kink
+    h = 1e-6
+    fd = (lr.gelu(z + h) - lr.gelu(z - h)) / (2 * h)
+    assert np.allclose(lr.gelu_grad(z), fd, rtol=1e-6, atol=1e-7)
+
+
 def test_bf16_rounding():
     x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 2 ** -9, 1.0 + 3 * 2 ** -9, -2.5], dtype=np.float32)
     # 7 stored mantissa bits: 1 + 2^-8 is a tie and goes to even (1),
