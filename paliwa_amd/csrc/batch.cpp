@@ -268,6 +268,7 @@ int execute(paliwa_batch *b, char const *wavelet_name, bool forward,
     return fail(PALIWA_ERR_INVALID_ARGUMENT,
                 "the plan belongs to device " + std::to_string(b->device));
   hipStream_t const s = static_cast<hipStream_t>(stream);
+  TraceRange const range(forward ? "paliwa batch hierarchize" : "paliwa batch dehierarchize");
   if (b->use_graphs) {
     hipGraphExec_t exec = nullptr;
     {

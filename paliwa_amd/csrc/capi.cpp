@@ -31,6 +31,7 @@ int named_transform(void *data, int32_t dtype, int32_t ndim,
     // std::map::at in paliwa/paliwa_transform.hpp:370, 397
     return fail(PALIWA_ERR_UNKNOWN_WAVELET, "map::at");
   for (int d = first_dim; d <= last_dim; ++d) {
+    TraceRange const range(axis_range_name(forward, d));
     SweepParams p = make_params(data, ndim, extents, level, d);
     if (forward)
       set_filters(p, w->forward, w->nforward);
@@ -103,6 +104,7 @@ int paliwa_transform_in_path(void *data, int32_t dtype, int32_t ndim,
     p.shift[i] = static_cast<int8_t>(level[dim] - levels[i]);
   }
   set_filters(p, filters, nfilters);
+  TraceRange const range("paliwa transform_in");
   rc = sweep(dtype, p, path, static_cast<hipStream_t>(stream));
   if (rc == PALIWA_OK)
     paliwa_hip::last_error().clear();

@@ -743,6 +743,7 @@ int paliwa_combine_gather(paliwa_combine *plan, void *sparse, void *stream) {
   if (sparse == nullptr)
     return fail(PALIWA_ERR_INVALID_ARGUMENT, "sparse is NULL");
   auto s = static_cast<hipStream_t>(stream);
+  TraceRange const range("paliwa combine gather");
   if (plan->dtype == PALIWA_FLOAT64)
     gather_dispatch<double>(plan->ndim, plan->args, plan->nblocks, sparse, s);
   else
@@ -779,6 +780,7 @@ int paliwa_combine_scatter(const paliwa_combine *plan, const void *sparse,
   FullExtents fe{};
   for (int d = 0; d < kCombineDims; ++d)
     fe.e[d] = ext[d];
+  TraceRange const range("paliwa combine scatter");
   if (plan->dtype == PALIWA_FLOAT64)
     scatter_dispatch<double>(D, plan->args, blocks, sparse, full, chunks, fe, s);
   else

@@ -8,6 +8,20 @@
 
 namespace paliwa_hip {
 
+char const *axis_range_name(bool forward, int dim) {
+  static std::string const *names = [] {
+    auto *n = new std::string[2 * PALIWA_MAX_DIMS];
+    for (int d = 0; d < PALIWA_MAX_DIMS; ++d) {
+      n[d] = "paliwa dehierarchize axis " + std::to_string(d);
+      n[PALIWA_MAX_DIMS + d] = "paliwa hierarchize axis " + std::to_string(d);
+    }
+    return n;
+  }();
+  if (dim < 0 || dim >= PALIWA_MAX_DIMS)
+    return "paliwa sweep";
+  return names[(forward ? PALIWA_MAX_DIMS : 0) + dim].c_str();
+}
+
 std::string &last_error() {
   static thread_local std::string message;
   return message;

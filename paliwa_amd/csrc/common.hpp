@@ -3,9 +3,22 @@
 
 #include "lifting.hpp"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include <string>
 
 namespace paliwa_hip {
+
+// A roctx range (rocprofv3 --marker-trace shows it on the host timeline
+// with the kernels launched inside it) for the lifetime of the object.
+struct TraceRange {
+  explicit TraceRange(char const *name) { roctxRangePushA(name); }
+  ~TraceRange() { roctxRangePop(); }
+  TraceRange(TraceRange const &) = delete;
+  TraceRange &operator=(TraceRange const &) = delete;
+};
+// "paliwa hierarchize axis d" / "paliwa dehierarchize axis d" (d < 26).
+char const *axis_range_name(bool forward, int dim);
 
 // Message of the last failed call on this thread.
 std::string &last_error();

@@ -258,3 +258,17 @@ def test_checked_storage_validation():
               _lib.i32_array([4]), 1, good, 1, None) == _lib.PALIWA_ERR_LEVEL
     assert fn(FAKE, 0, 2, st, _lib.i32_array([3, 3]), _lib.i32_array([4, 4]), 2,
               _lib.i32_array([3]), 1, good, 1, None) == _lib.PALIWA_ERR_INVALID_ARGUMENT
+
+
+def test_entry_points_open_roctx_ranges():
+    """The library brackets its entry points in roctx ranges (named per axis
+    for the named transforms), which rocprofv3 --marker-trace shows."""
+    import subprocess
+
+    out = subprocess.run(["nm", "-D", _lib.LIBRARY_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    assert "roctxRangePushA" in out and "roctxRangePop" in out
+    strings = subprocess.run(["strings", _lib.LIBRARY_PATH], capture_output=True, text=True,
+                             check=True).stdout
+    for name in ("paliwa hierarchize axis ", "paliwa batch hierarchize", "paliwa combine gather"):
+        assert name in strings, name
