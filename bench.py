@@ -61,6 +61,9 @@ def parse():
     ap.add_argument("--ct-replays", type=int, default=0,
                     help="only build the CT set's batched plan and replay its forward "
                          "direction this many times (kernel traces of the schedule)")
+    ap.add_argument("--cold-probe", action="store_true",
+                    help="only time the 257^3 sweeps warm, after a 512 MB write flush "
+                         "of the Infinity Cache and after a 512 MB read flush (counter runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     return ap.parse_args()
@@ -75,6 +78,48 @@ def copy_probe(torch):
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
                       "launches": 5}))
+
+
+def cold_probe(torch, reps=5):
+    """The six forward/inverse sweeps of the 257^3 biorthogonal grid, each
+    timed alone (median of `reps`) in three cache states: warm (the grid was
+    just swept), after writing a 512 MB buffer (the Infinity Cache then holds
+    dirty lines of another buffer, which the sweep evicts to HBM), and after
+    reading a 512 MB buffer (clean lines: eviction costs nothing).  The states
+    run in the order warm, write flush, read flush for every sweep, so a
+    counter run sees them as consecutive launches of the sweep kernels."""
+    import paliwa_amd as pw
+    from paliwa_amd import fields
+
+    grid = fields.fourier_modes(LEVEL, modes=16, kmax=32, seed=1, device="cuda")
+    level, lmin = list(LEVEL), [0] * len(LEVEL)
+    flush = torch.ones(512 * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
+    sink = torch.zeros(1, dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    states = {"warm": lambda: None,
+              "write_flush": lambda: flush.fill_(1.0),
+              "read_flush": lambda: torch.sum(flush, dim=0, out=sink)}
+    out = {}
+    for fwd in (True, False):
+        fn = pw.hierarchize_in if fwd else pw.dehierarchize_in
+        for d in range(len(LEVEL)):
+            times = {k: [] for k in states}
+            fn(d, grid, level, lmin, level, WAVELET)  # warm-up
+            for _ in range(reps):
+                for name, prep in states.items():
+                    if name == "warm":
+                        fn(d, grid, level, lmin, level, WAVELET)
+                    prep()
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    fn(d, grid, level, lmin, level, WAVELET)
+                    e1.record(stream)
+                    torch.cuda.synchronize()
+                    times[name].append(e0.elapsed_time(e1))
+            key = f"{'fwd' if fwd else 'inv'}_dim{d}"
+            out[key] = {k: round(sorted(v)[len(v) // 2], 4) for k, v in times.items()}
+    print(json.dumps({"cold_probe_ms": out, "bytes_per_sweep": 2 * grid.numel() * 8}))
 
 
 def partial_benchmark(torch, ranks=8, reps=20):
@@ -591,6 +636,9 @@ def main():
     if args.ct_replays:
         ct_replays(torch, args.ct_replays)
         return
+    if args.cold_probe:
+        cold_probe(torch)
+        return
 
     import paliwa_amd as pw
     from paliwa_amd import fields
@@ -676,24 +724,34 @@ def main():
     dom_ms = avg[dominant]
     achieved = bytes_per_sweep / (dom_ms * 1e-3) / 1e9
 
-    # cold: flush the Infinity Cache (write 512 MB) before each timed sweep
-    flush = torch.empty(512 * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
-    cold = {}
+    # cold: the Infinity Cache holds another 512 MB buffer when the sweep
+    # starts -- read before it (clean lines: the grid comes from HBM) or
+    # written before it (dirty lines, which the sweep's own traffic evicts to
+    # HBM on top of its reads and writes)
+    flush = torch.ones(512 * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
+    sink = torch.zeros(1, dtype=torch.float32, device="cuda")
+    cold, cold_dirty = {}, {}
     for fwd in (True, False):
         fn = pw.hierarchize_in if fwd else pw.dehierarchize_in
         for d in range(D):
-            times = []
+            times = {"clean": [], "dirty": []}
             for _ in range(3):
-                flush.fill_(1.0)
-                e0 = torch.cuda.Event(enable_timing=True)
-                e1 = torch.cuda.Event(enable_timing=True)
-                e0.record(stream)
-                fn(d, grid, level, lmin, level, WAVELET)
-                e1.record(stream)
-                torch.cuda.synchronize()
-                times.append(e0.elapsed_time(e1))
-            cold[("fwd" if fwd else "inv", d)] = sorted(times)[1]
+                for kind in ("clean", "dirty"):
+                    if kind == "clean":
+                        torch.sum(flush, dim=0, out=sink)
+                    else:
+                        flush.fill_(1.0)
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    fn(d, grid, level, lmin, level, WAVELET)
+                    e1.record(stream)
+                    torch.cuda.synchronize()
+                    times[kind].append(e0.elapsed_time(e1))
+            cold[("fwd" if fwd else "inv", d)] = sorted(times["clean"])[1]
+            cold_dirty[("fwd" if fwd else "inv", d)] = sorted(times["dirty"])[1]
     cold_step_ms = sum(cold.values())
+    cold_dirty_step_ms = sum(cold_dirty.values())
     del flush
 
     # practical streaming ceiling for this footprint: an in-place elementwise
@@ -759,6 +817,9 @@ def main():
             "cold_GBps": round(bytes_per_step / (cold_step_ms * 1e-3) / 1e9, 2),
             "per_kernel_ms": {f"{k[0]}_dim{k[1]}": round(v, 4) for k, v in sorted(avg.items())},
             "cold_per_kernel_ms": {f"{k[0]}_dim{k[1]}": round(v, 4) for k, v in sorted(cold.items())},
+            "cold_dirty_ms_per_step": round(cold_dirty_step_ms, 4),
+            "cold_dirty_per_kernel_ms": {f"{k[0]}_dim{k[1]}": round(v, 4)
+                                         for k, v in sorted(cold_dirty.items())},
             "round_trip_rel_err": rt_once,
             "accumulated_rel_err": {"round_trips": n_round_trips, "max_rel": rt_err},
         },
@@ -771,10 +832,17 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "residency": "warm: the 136 MB grid stays in the 256 MB Infinity Cache "
-                         "between sweeps; cold: the cache is flushed before the sweep",
+                         "between sweeps; cold: the cache holds another 512 MB buffer, "
+                         "read before the sweep (clean); cold_dirty: written before it, "
+                         "so the sweep also evicts dirty lines to HBM",
             "cold": {
                 "achieved": round(bytes_per_sweep / (cold[dominant] * 1e-3) / 1e9, 2),
                 "frac": round(bytes_per_sweep / (cold[dominant] * 1e-3) / 1e9
+                              / HBM_PEAK_GBS, 4),
+            },
+            "cold_dirty": {
+                "achieved": round(bytes_per_sweep / (cold_dirty[dominant] * 1e-3) / 1e9, 2),
+                "frac": round(bytes_per_sweep / (cold_dirty[dominant] * 1e-3) / 1e9
                               / HBM_PEAK_GBS, 4),
             },
         },
