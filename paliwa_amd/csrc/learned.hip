@@ -2,7 +2,8 @@
 // integer lifting transform whose predict step at levels >= 3 adds to the
 // interpolet (hat) predict the output of a small MLP
 //     8 -> 32 -> 32 -> 1, GELU, bf16 weights and activations, fp32 accumulate,
-// evaluated with bf16 MFMA (v_mfma_f32_16x16x16_bf16), whose inputs are the 4
+// evaluated with bf16 MFMA (v_mfma_f32_16x16x16_bf16 for the first layer,
+// v_mfma_f32_16x16x32_bf16 for the other two), whose inputs are the 4
 // coarse neighbours on each side of the predicted point along the lifted
 // axis, divided by their max-abs.  The interpolet predict is the fallback.
 //
@@ -43,6 +44,11 @@ constexpr int kTile = 16;    // lines per tile
 constexpr int kLogTile = 4;
 constexpr int kHidden = 32;
 constexpr int kInputs = 8;
+// network groups of 16 points interleaved per wavefront in the transform
+#ifndef PALIWA_LEARNED_GROUPS
+#define PALIWA_LEARNED_GROUPS 2
+#endif
+constexpr int kGroupsInFlight = PALIWA_LEARNED_GROUPS;
 // Packed fp32 parameter layout.
 constexpr int kW1 = 0;                         // [32][8]
 constexpr int kB1 = kW1 + kHidden * kInputs;   // [32]
@@ -65,158 +71,188 @@ __device__ __forceinline__ float from_bf16(short b) {
 __device__ __forceinline__ float round_bf16(float x) {
   return from_bf16(to_bf16(x));
 }
-// four fp32 -> one bf16x4 fragment, two values per v_cvt_pk_bf16_f32 (the
-// same rounding as to_bf16)
+// bf16 pairs: one v_cvt_pk_bf16_f32 (the same rounding as to_bf16)
 typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 typedef float f32x2_t __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ bf16x4 pack_bf16(f32x4 v) {
-  bf16x2_t const lo = __builtin_convertvector(f32x2_t{v[0], v[1]}, bf16x2_t);
-  bf16x2_t const hi = __builtin_convertvector(f32x2_t{v[2], v[3]}, bf16x2_t);
-  u32x2_t const w = {__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi)};
-  return __builtin_bit_cast(bf16x4, w);
-}
 
-// GELU without transcendentals: z * g(t), t = z clamped to [-kGeluClamp,
-// kGeluClamp], g(t) = 0.5 + t P(t^2) clamped to [0, 1], P of degree 4: a
-// minimax fit (over all z) to the tanh form
-//   gelu(z) = 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))),
-// within 1.28e-3 of it everywhere, below the bf16 rounding the activations
-// get once |gelu| > 0.3 (the gate reaches 1 at the clamp, so z * g is z
-// beyond it, and 0 below -kGeluClamp).  Eight full-rate fp32 instructions
-// (a median, a square, five multiply-adds, a product) in place of an
-// exponential and a reciprocal, which issue at a quarter of the rate.  The
-// transform's both directions and the training kernels all use these
-// functions, so the network the inverse evaluates is bit for bit the one the
-// forward evaluated.
-constexpr float kGeluClamp = 3.4513f;
-constexpr float kGeluP0 = 3.93313942e-01f, kGeluP1 = -5.90118521e-02f,
-                kGeluP2 = 6.50023659e-03f, kGeluP3 = -3.92342173e-04f,
-                kGeluP4 = 9.70034432e-06f;
-__device__ __forceinline__ float gelu_poly(float u) { // P(u)
-  float p = __builtin_fmaf(kGeluP4, u, kGeluP3);
-  p = __builtin_fmaf(p, u, kGeluP2);
-  p = __builtin_fmaf(p, u, kGeluP1);
-  return __builtin_fmaf(p, u, kGeluP0);
+// GELU without transcendentals, evaluated two values per instruction in
+// packed fp16:
+//   gelu(z) = z * g(z),  g = clamp(0.5 + s Q(s^2 - 3/8), 0, 1),  s = fp16(z / 4),
+// Q of degree 4 with fp16 coefficients, every step an fp16 multiply-add
+// (one rounding each), z * g in fp32, then the bf16 rounding every activation
+// gets.  In exact arithmetic g is a monotone minimax fit to the gate of the
+// tanh form
+//   gelu(z) = 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3)))
+// within 1.07e-3 of it in z * g; with the fp16 roundings, within 2.04e-3 for
+// every fp16 s (below the bf16 rounding of activations above 0.5 in
+// magnitude).  g reaches 1 at z = 3.378 and stays there (0 below -3.378):
+// the polynomial's leading coefficient is positive and, in fp16, s^2 - 3/8
+// and Q overflow to +inf for large |s|, which the clamp maps to 1 or 0, so no
+// clamp of s is needed.  Per pair of activations: a conversion to fp16, five
+// packed multiply-adds for the polynomial and one (with the [0, 1] clamp) for
+// g, two fp32 products (v_fma_mix_f32 reads g's fp16 halves), one bf16 pack.
+// The transform's both directions and the training kernels all evaluate g
+// with these fp16 operations, so the network the inverse evaluates is bit for
+// bit the one the forward evaluated.
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+constexpr float kGateScale = 0.25f; // s = z * kGateScale
+constexpr float kGateShift = 0.375f;
+constexpr float kGateQ0 = 0.8046875f, kGateQ1 = -0.9580078125f, kGateQ2 = 1.541015625f,
+                kGateQ3 = -2.6484375f, kGateQ4 = 2.87109375f; // exact fp16 values
+
+template <typename V>
+__device__ __forceinline__ V splat(float x) {
+  return static_cast<V>(static_cast<_Float16>(x));
 }
+template <>
+__device__ __forceinline__ f16x2 splat<f16x2>(float x) {
+  _Float16 const h = static_cast<_Float16>(x);
+  return f16x2{h, h};
+}
+template <>
+__device__ __forceinline__ f16x4 splat<f16x4>(float x) {
+  _Float16 const h = static_cast<_Float16>(x);
+  return f16x4{h, h, h, h};
+}
+// V = _Float16 (the training kernels) or f16x4 (the transform): the same
+// fp16 operations element by element, so both give bit-identical gates.
+template <typename V>
+__device__ __forceinline__ V gate(V s) {
+  V const w = __builtin_elementwise_fma(s, s, splat<V>(-kGateShift));
+  V p = __builtin_elementwise_fma(splat<V>(kGateQ4), w, splat<V>(kGateQ3));
+  p = __builtin_elementwise_fma(p, w, splat<V>(kGateQ2));
+  p = __builtin_elementwise_fma(p, w, splat<V>(kGateQ1));
+  p = __builtin_elementwise_fma(p, w, splat<V>(kGateQ0));
+  V const g = __builtin_elementwise_fma(s, p, splat<V>(0.5f));
+  return __builtin_elementwise_min(__builtin_elementwise_max(g, splat<V>(0.0f)),
+                                   splat<V>(1.0f));
+}
+// gelu of one fp32 pre-activation (training kernels)
 __device__ __forceinline__ float gelu(float z) {
-  float const t = __builtin_amdgcn_fmed3f(z, -kGeluClamp, kGeluClamp);
-  float const g = __builtin_amdgcn_fmed3f(__builtin_fmaf(t, gelu_poly(t * t), 0.5f), 0.0f, 1.0f);
-  return z * g;
+  _Float16 const g = gate(static_cast<_Float16>(z * kGateScale));
+  return z * static_cast<float>(g);
 }
-// d/dz of gelu above: g(t) + z g'(t) where neither clamp holds,
-// g'(t) = P(u) + 2 u P'(u), u = t^2.
+// d/dz of z * g(z) for the exact-arithmetic g (straight through the fp16
+// roundings): g + z g'(z) where 0 < raw g < 1, g'(z) = (Q(w) + 2 s^2 Q'(w)) / 4.
 __device__ __forceinline__ float gelu_grad(float z) {
-  float const t = __builtin_amdgcn_fmed3f(z, -kGeluClamp, kGeluClamp);
-  float const u = t * t;
-  float const p = gelu_poly(u);
-  float const raw = __builtin_fmaf(t, p, 0.5f);
+  float const s = z * kGateScale;
+  float const w = s * s - kGateShift;
+  float p = __builtin_fmaf(kGateQ4, w, kGateQ3);
+  p = __builtin_fmaf(p, w, kGateQ2);
+  p = __builtin_fmaf(p, w, kGateQ1);
+  p = __builtin_fmaf(p, w, kGateQ0);
+  float const raw = __builtin_fmaf(s, p, 0.5f);
+  float dp = __builtin_fmaf(4.0f * kGateQ4, w, 3.0f * kGateQ3);
+  dp = __builtin_fmaf(dp, w, 2.0f * kGateQ2);
+  dp = __builtin_fmaf(dp, w, kGateQ1);
+  float const dg = kGateScale * __builtin_fmaf(2.0f * s * s, dp, p);
   float const g = __builtin_amdgcn_fmed3f(raw, 0.0f, 1.0f);
-  float dp = __builtin_fmaf(4.0f * kGeluP4, u, 3.0f * kGeluP3);
-  dp = __builtin_fmaf(dp, u, 2.0f * kGeluP2);
-  dp = __builtin_fmaf(dp, u, kGeluP1);
-  float const dg = __builtin_fmaf(2.0f * u, dp, p);
-  bool const inside = fabsf(z) < kGeluClamp && raw > 0.0f && raw < 1.0f;
-  return inside ? __builtin_fmaf(z, dg, g) : g;
+  return raw > 0.0f && raw < 1.0f ? __builtin_fmaf(z, dg, g) : g;
 }
 
 // Per-lane register fragments of the network for the MFMA evaluation.
 // Transposed formulation: each layer computes H^T = W * X^T with A = weight
 // rows (units) x K and B = X^T (K x 16 points), so the D layout of one layer
-// (lane l: units 4*(l/16)+j, point l%16) is the B layout of the next.
+// (lane l: units 4*(l/16)+j of a 16-unit row block, point l%16) is the B
+// layout of the next.  Layer 1 runs v_mfma_f32_16x16x16_bf16 (K = the 8
+// inputs, padded with zero weights), layers 2 and 3 gfx950's
+// v_mfma_f32_16x16x32_bf16, whose B fragment (lane l: 8 consecutive k of
+// point l%16) is the two row blocks' D fragments side by side: lane group
+// q = l/16 holds units 4q..4q+3 and 16+4q..16+4q+3, the k order kunit()
+// gives the weights.  Biases are the accumulators' initial values (fp32).
+// Layers 1 and 2 run on weights and biases scaled by 1/4 and the output layer
+// on weights scaled by 4 (exact in bf16 and fp32), so the accumulators hold
+// z/4 = s before its fp16 rounding, the products z/4 * g are the
+// activations / 4, and the output is r; every rounding scales with them, so
+// each value is exactly the unscaled network's.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 struct Net {
-  bf16x4 w1[2];    // A fragments of [W1 | b1 | 0] (K padded 8 -> 16), 2 row blocks
-  bf16x4 w2[2][2]; // A fragments of W2, [row block][k block]
-  float b2[2][4];  // bias of the 4 units this lane holds per row block
-  bf16x4 w3[2];    // A fragments of the output weights, the same in every row, [k block]
+  bf16x4 w1[2]; // A fragments of W1 / 4 per row block (k >= 8: 0)
+  f32x4 b1[2];  // b1 / 4 of the lane's 4 D units per row block
+  bf16x8 w2[2]; // A fragments of W2 per row block, k order kunit()
+  f32x4 b2[2];  // b2 / 4
+  bf16x8 w3;    // 4 * output weights in every row, k order kunit()
   float b3;
 };
+
+// hidden unit of slot j (0..7) of lane group q in a 16x16x32 fragment
+__device__ __forceinline__ int kunit(int q, int j) {
+  return j < 4 ? 4 * q + j : 16 + 4 * q + (j - 4);
+}
 
 __device__ __forceinline__ void load_net(float const *prm, Net &net) {
   int const lane = threadIdx.x & 63;
   int const row = lane & 15;
-  int const kq = 4 * (lane >> 4);
+  int const q = lane >> 4;
 #pragma unroll
   for (int blk = 0; blk < 2; ++blk) {
     int const unit = 16 * blk + row;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      int const k = kq + j;
-      float w = 0.0f;
-      if (k < kInputs)
-        w = prm[kW1 + unit * kInputs + k];
-      else if (k == kInputs)
-        w = prm[kB1 + unit];
-      net.w1[blk][j] = to_bf16(w);
+      int const k = 4 * q + j;
+      net.w1[blk][j] = to_bf16(k < kInputs ? prm[kW1 + unit * kInputs + k] * kGateScale : 0.0f);
+      net.b1[blk][j] = prm[kB1 + 16 * blk + 4 * q + j] * kGateScale;
+      net.b2[blk][j] = prm[kB2 + 16 * blk + 4 * q + j] * kGateScale;
     }
 #pragma unroll
-    for (int kb = 0; kb < 2; ++kb)
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        net.w2[blk][kb][j] = to_bf16(prm[kW2 + unit * kHidden + 16 * kb + kq + j]);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      int const u = 16 * blk + kq + j; // D-layout unit of this lane
-      net.b2[blk][j] = prm[kB2 + u];
-      net.w3[blk][j] = to_bf16(prm[kW3 + u]); // k = unit u of k block blk
-    }
+    for (int j = 0; j < 8; ++j)
+      net.w2[blk][j] = static_cast<__bf16>(prm[kW2 + unit * kHidden + kunit(q, j)]);
   }
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    net.w3[j] = static_cast<__bf16>(prm[kW3 + kunit(q, j)] * 4.0f);
   net.b3 = prm[kB3];
 }
 
-// MLP on NB independent groups of 16 points of one wavefront (NB chains in
-// flight hide each other's MFMA and exponential latencies).  Lane l supplies
-// inputs x[4*(l/16) .. +3] of point l%16 of each group (lanes 32..63: the
-// bias input 1 at k = 8, zeros elsewhere); out[b] is the network output of
-// point l%16 of group b, in every lane.  The arithmetic per group does not
-// depend on NB.
+// One D fragment of pre-activations s = z/4 (fp32) -> the 4 activations / 4,
+// bf16, packed as one half of the next layer's B fragment.  The gate runs on
+// an fp16 4-vector: two packed instructions per step, independent of each
+// other, so neither waits on the other's result.
+__device__ __forceinline__ u32x2_t activate(f32x4 d) {
+  f16x4 const g = gate(__builtin_convertvector(d, f16x4));
+  // z/4 * g in fp32, one rounding (v_fma_mix_f32 with a zero addend)
+  float const h0 = __builtin_fmaf(d[0], static_cast<float>(g[0]), 0.0f);
+  float const h1 = __builtin_fmaf(d[1], static_cast<float>(g[1]), 0.0f);
+  float const h2 = __builtin_fmaf(d[2], static_cast<float>(g[2]), 0.0f);
+  float const h3 = __builtin_fmaf(d[3], static_cast<float>(g[3]), 0.0f);
+  return u32x2_t{__builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2_t{h0, h1}, bf16x2_t)),
+                 __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2_t{h2, h3}, bf16x2_t))};
+}
+
+__device__ __forceinline__ bf16x8 side_by_side(u32x2_t const (&h)[2]) {
+  return __builtin_bit_cast(bf16x8, u32x4{h[0][0], h[0][1], h[1][0], h[1][1]});
+}
+
+// The network on NB groups of 16 points of one wavefront, layer by layer
+// across the groups (one group's MFMAs run while another's activations
+// issue): lane l supplies inputs x[4*(l/16) .. +3] of point l%16 of each
+// group (lane groups 2 and 3: any finite values, their weights are 0).
+// out[b] = r of point l%16 of group b, in every lane.  The arithmetic
+// of a group does not depend on NB.
 template <int NB>
-__device__ __forceinline__ void mlp16(Net const &net, bf16x4 const (&xin)[NB],
-                                      float (&out)[NB]) {
-  f32x4 const zero = {0.0f, 0.0f, 0.0f, 0.0f};
-  bf16x4 h1[NB][2];
+__device__ __forceinline__ void mlp16(Net const &net, bf16x4 const (&x)[NB], float (&out)[NB]) {
+  u32x2_t h1[NB][2], h2[NB][2];
 #pragma unroll
   for (int blk = 0; blk < 2; ++blk)
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      f32x4 const d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w1[blk], xin[b], zero, 0, 0, 0);
-      h1[b][blk] = pack_bf16(f32x4{gelu(d[0]), gelu(d[1]), gelu(d[2]), gelu(d[3])});
-    }
-  // layer 2, then the output layer on the matrix cores: A = the output
-  // weights in every row, so every row of D (every lane) holds the output of
-  // its point; the output bias is the accumulator's initial value
-  bf16x4 h2[NB][2];
+    for (int b = 0; b < NB; ++b)
+      h1[b][blk] = activate(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w1[blk], x[b],
+                                                                      net.b1[blk], 0, 0, 0));
 #pragma unroll
-  for (int blk = 0; blk < 2; ++blk) {
-    // the layer-2 bias is the accumulator's initial value
-    f32x4 const bias = {net.b2[blk][0], net.b2[blk][1], net.b2[blk][2], net.b2[blk][3]};
+  for (int blk = 0; blk < 2; ++blk)
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w2[blk][0], h1[b][0], bias, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w2[blk][1], h1[b][1], d, 0, 0, 0);
-      h2[b][blk] = pack_bf16(f32x4{gelu(d[0]), gelu(d[1]), gelu(d[2]), gelu(d[3])});
-    }
-  }
+    for (int b = 0; b < NB; ++b)
+      h2[b][blk] = activate(__builtin_amdgcn_mfma_f32_16x16x32_bf16(
+          net.w2[blk], side_by_side(h1[b]), net.b2[blk], 0, 0, 0));
   f32x4 const b3 = {net.b3, net.b3, net.b3, net.b3};
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w3[0], h2[b][0], b3, 0, 0, 0);
-    d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(net.w3[1], h2[b][1], d, 0, 0, 0);
-    out[b] = d[0];
-  }
-}
-
-// Inputs of this lane's 4 MLP slots for one predicted point: the 8 features
-// (lanes 0..31), the bias input 1 / zeros (lanes 32..63).
-__device__ __forceinline__ bf16x4 mlp_inputs(float const (&v)[8]) {
-  int const q = (threadIdx.x & 63) >> 4;
-  bf16x4 x;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    float const u = q < 2 ? v[4 * q + j] : ((q == 2 && j == 0) ? 1.0f : 0.0f);
-    x[j] = to_bf16(u);
-  }
-  return x;
+  for (int b = 0; b < NB; ++b)
+    out[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(net.w3, side_by_side(h2[b]), b3, 0, 0,
+                                                     0)[0];
 }
 
 // Network inputs of one predicted point from its 8 coarse neighbours
@@ -236,35 +272,43 @@ enum Features : int32_t { kFeaturesRaw = 0, kFeaturesDetrended = 1 };
 // point between two monotone runs of neighbours (a step between c_3 and c_4
 // gives |x - hat| <= max|D| / 12), and a tight bound limits the growth of
 // the details, hence of the coarse values, where the network is wrong.
-__device__ __forceinline__ float correction_bound(int32_t kind) {
-  return kind == kFeaturesDetrended ? 1.0f : 4.0f;
+template <int32_t KIND>
+constexpr float correction_bound() {
+  return KIND == kFeaturesDetrended ? 1.0f : 4.0f;
 }
 
 // Features in fp32: a deterministic function of the coarse values (the
 // forward and the inverse run these same instructions on the same inputs),
 // which is all the integer lifting step needs to stay exactly invertible.
-__device__ __forceinline__ float features(int const (&c)[8], int32_t kind,
-                                          float (&u)[8], float &norm) {
+// Returns the correction unit; |u| <= 1 + 2^-22 (the approximate reciprocal
+// can round max|f| * inv above 1), which the bf16 rounding of the network
+// inputs takes back to 1.
+template <int32_t KIND>
+__device__ __forceinline__ float features(int const (&c)[8], float (&u)[8], float &norm) {
   float f[8];
-  float const mid = static_cast<float>(c[3]) + static_cast<float>(c[4]);
-  float const slope = static_cast<float>(c[4]) - static_cast<float>(c[3]);
+  if constexpr (KIND == kFeaturesDetrended) {
+    float const mid = static_cast<float>(c[3]) + static_cast<float>(c[4]);
+    float const slope = static_cast<float>(c[4]) - static_cast<float>(c[3]);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    float const ck = static_cast<float>(c[k]);
-    f[k] = kind == kFeaturesDetrended
-               ? (2.0f * ck - mid) - slope * static_cast<float>(2 * k - 7)
-               : ck;
+    for (int k = 0; k < 8; ++k) // (2 c_k - mid) - slope (2k - 7), two roundings
+      f[k] = k == 3 || k == 4
+                 ? 0.0f
+                 : __builtin_fmaf(-slope, static_cast<float>(2 * k - 7),
+                                  __builtin_fmaf(2.0f, static_cast<float>(c[k]), -mid));
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      f[k] = static_cast<float>(c[k]);
   }
   norm = 0.0f;
 #pragma unroll
   for (int k = 0; k < 8; ++k)
     norm = fmaxf(norm, fabsf(f[k]));
   float const inv = norm > 0.0f ? __builtin_amdgcn_rcpf(norm) : 0.0f;
-  // |u| <= 1 (the approximate reciprocal can round max|f| * inv to 1 + 2^-23)
 #pragma unroll
-  for (int k = 0; k < 8; ++k)
-    u[k] = fminf(fmaxf(f[k] * inv, -1.0f), 1.0f);
-  return kind == kFeaturesDetrended ? 0.5f * norm : norm; // correction unit
+  for (int k = 0; k < 8; ++k) // detrended D_3 = D_4 = 0 whatever inv is
+    u[k] = KIND == kFeaturesDetrended && (k == 3 || k == 4) ? 0.0f : f[k] * inv;
+  return KIND == kFeaturesDetrended ? 0.5f * norm : norm;
 }
 
 struct LearnedArgs {
@@ -307,18 +351,29 @@ __device__ __forceinline__ int64_t xcd_tile(uint32_t bid, uint32_t nb) {
                  : int64_t(r) * (q + 1) + int64_t(xcd - r) * q + local;
 }
 
+// Orders one wavefront's LDS accesses around an exchange between its lanes
+// (a wavefront's LDS instructions execute in program order; this keeps the
+// compiler from moving loads and stores across it).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <bool STRIDED>
 __device__ __forceinline__ int lidx(int line, int r, int m) {
   return STRIDED ? r * kTile + line : line * m + r;
 }
 
-template <bool STRIDED>
+// KIND: the network's input features (kFeatures*).
+template <bool STRIDED, int32_t KIND>
 __global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
   extern __shared__ int32_t tile[];
   __shared__ int32_t overflow_flag;
+  __shared__ u32x4 xfer[kThreads]; // network inputs, one 16-byte slot per lane
   int const tid = threadIdx.x;
   int const lane = tid & 63;
-  int const wave = tid >> 6;
+  int const wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   int const m = a.m;
   int const n = 1 << a.L;
   int64_t const lines = a.A * a.B;
@@ -383,7 +438,7 @@ __global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
       // tile's 16 lines (lane & 15 = line), so the neighbour reads of a batch
       // are 16 consecutive LDS words (strided layout) or words m apart (m odd)
       double l1_mlp = 0.0, l1_hat = 0.0;
-      float const bound = correction_bound(a.features);
+      constexpr float bound = correction_bound<KIND>();
       int const l = lane & (kTile - 1);
       int const q = lane >> 4;
       // A wavefront takes 4 odd point indices k at a time across the tile's
@@ -391,54 +446,73 @@ __global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
       // neighbours and computes its features (64 distinct points per
       // instruction; the neighbour reads of 16 lanes are 16 consecutive LDS
       // words in the strided layout, words m apart (m odd) otherwise).  The
-      // network then runs as 4 MFMA groups of 16 points: group g takes the
-      // bf16 features of lanes 16g..16g+15 by shuffles, and lane (q, l)
+      // network then runs as 4 MFMA groups of 16 points: lane (q, l) stores
+      // its point's 8 bf16 features in its 16-byte slot of the wave's LDS
+      // row, group g reads the B fragment of its 16 points back (lane
+      // (q', l): 8 bytes, features 4(q' & 1)..+3 of point (g, l); lane groups
+      // 2 and 3 read features too, against zero weights), and lane (q, l)
       // keeps the output of group q.
+      // Neighbours x -/+ 7s .. s wrap around the period only for k < 3 or
+      // k > cnt - 5; a step whose 4 indices k0..k0+3 avoid both (a test on
+      // wave-uniform values) addresses them at fixed offsets from x.
+      int const row_stride = STRIDED ? kTile : 1;
       for (int k0 = 4 * wave; k0 < cnt; k0 += 4 * (kThreads / 64)) {
         int const k = k0 + q;
         bool const valid = l < nl && k < cnt;
         int const kk = k < cnt ? k : k0;
         int const x = s + 2 * s * kk;
+        int const xi = lidx<STRIDED>(l, x, m);
         int c[8];
+        if (k0 >= 3 && k0 + 3 <= cnt - 5) {
+          int32_t const *px = tile + xi;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          int const off = (2 * (3 - j) + 1) * s; // 7s, 5s, 3s, s
-          c[j] = tile[lidx<STRIDED>(l, (x - off) & nmask, m)];
-          c[7 - j] = tile[lidx<STRIDED>(l, (x + off) & nmask, m)];
+          for (int j = 0; j < 4; ++j) {
+            int const off = (2 * (3 - j) + 1) * s * row_stride; // 7s, 5s, 3s, s rows
+            c[j] = px[-off];
+            c[7 - j] = px[off];
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            int const off = (2 * (3 - j) + 1) * s;
+            c[j] = tile[lidx<STRIDED>(l, (x - off) & nmask, m)];
+            c[7 - j] = tile[lidx<STRIDED>(l, (x + off) & nmask, m)];
+          }
         }
         float u[8], norm;
-        float const unit = features(c, a.features, u, norm);
-        float r = 0.0f;
+        float const unit = features<KIND>(c, u, norm);
+        float r = 0.0f; // network output
         if (a.weights != nullptr) {
-          uint32_t word[4];
+          u32x4 word;
 #pragma unroll
           for (int j = 0; j < 4; ++j) // bf16 pairs, one v_cvt_pk_bf16_f32 each
             word[j] = __builtin_bit_cast(
                 uint32_t, __builtin_convertvector(f32x2_t{u[2 * j], u[2 * j + 1]}, bf16x2_t));
+          xfer[tid] = word;
+          wave_sync();
+          u32x2_t const *row = reinterpret_cast<u32x2_t const *>(xfer + (tid & ~63));
 #pragma unroll
-          for (int g = 0; g < 4; ++g) {
-            int const src = 16 * g + l;
-            uint32_t w[4];
+          for (int g0 = 0; g0 < 4; g0 += kGroupsInFlight) {
+            bf16x4 xs[kGroupsInFlight];
+            float out[kGroupsInFlight];
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-              w[j] = static_cast<uint32_t>(__shfl(static_cast<int>(word[j]), src, 64));
-            // lanes q = 0, 1: inputs 4q..4q+3; q = 2: the bias input 1; q = 3: 0
-            uint32_t const lo = q == 0 ? w[0] : q == 1 ? w[2] : q == 2 ? 0x3f80u : 0u;
-            uint32_t const hi = q == 0 ? w[1] : q == 1 ? w[3] : 0u;
-            bf16x4 const xs[1] = {__builtin_bit_cast(bf16x4, u32x2_t{lo, hi})};
-            float out[1];
-            mlp16<1>(net, xs, out);
-            r = g == q ? out[0] : r;
+            for (int b = 0; b < kGroupsInFlight; ++b)
+              xs[b] = __builtin_bit_cast(bf16x4, row[2 * (16 * (g0 + b) + l) + (q & 1)]);
+            mlp16<kGroupsInFlight>(net, xs, out);
+#pragma unroll
+            for (int b = 0; b < kGroupsInFlight; ++b)
+              r = g0 + b == q ? out[b] : r;
           }
+          wave_sync(); // every lane's reads before the next iteration's stores
         }
         int64_t const hat = (int64_t(c[3]) + c[4] + 1) >> 1;
-        // within +-2^30 the correction converts to int32 in one instruction;
-        // a larger one would overflow the int32 detail anyway (flagged below)
-        float corr = unit * r;
-        corr = fminf(fmaxf(corr, -bound * unit), bound * unit);
-        corr = fminf(fmaxf(corr, -1073741824.0f), 1073741824.0f);
+        // clamped to +-bound units and to +-2^30: within that the correction
+        // converts to int32 in one instruction; a larger one would overflow
+        // the int32 detail anyway (flagged below)
+        float const lim = __builtin_amdgcn_fmed3f(bound * unit, 0.0f, 1073741824.0f);
+        float const corr = __builtin_amdgcn_fmed3f(unit * r, -lim, lim);
         int64_t const pm = hat + __float2int_rn(corr);
-        int32_t const xv = tile[lidx<STRIDED>(l, x, m)];
+        int32_t const xv = tile[xi];
         if (!valid)
           continue;
         if (want_pairs && !a.inverse) {
@@ -450,7 +524,7 @@ __global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
             if (slot < a.pair_capacity) {
               float *pr = a.pairs + size_t(slot) * 10;
               for (int j = 0; j < 8; ++j)
-                pr[j] = u[j];
+                pr[j] = fminf(fmaxf(u[j], -1.0f), 1.0f);
               // the predictor clamps its correction to +-bound units, so
               // targets beyond that cannot be fitted: clamp them too
               pr[8] = fminf(fmaxf(static_cast<float>(int64_t(xv) - hat) / unit, -bound), bound);
@@ -466,7 +540,7 @@ __global__ __launch_bounds__(kThreads) void learned_sweep(LearnedArgs a) {
         int64_t const out = a.inverse ? int64_t(xv) + pred : int64_t(xv) - pred;
         if (out > INT32_MAX || out < INT32_MIN)
           overflow_flag = 1;
-        tile[lidx<STRIDED>(l, x, m)] = static_cast<int32_t>(out);
+        tile[xi] = static_cast<int32_t>(out);
       }
       if (want_stats) {
         atomicAdd(&a.stats[2 * lev + 0], l1_mlp);
@@ -560,14 +634,8 @@ __global__ __launch_bounds__(kThreads) void mlp_eval_kernel(
     bool const valid = p < count;
     bf16x4 x;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      float v;
-      if (q < 2)
-        v = valid ? inputs[size_t(p) * 8 + 4 * q + j] : 0.0f;
-      else
-        v = (q == 2 && j == 0) ? 1.0f : 0.0f;
-      x[j] = to_bf16(v);
-    }
+    for (int j = 0; j < 4; ++j) // lane groups 2, 3: zeros against zero weights
+      x[j] = to_bf16(q < 2 && valid ? inputs[size_t(p) * 8 + 4 * q + j] : 0.0f);
     bf16x4 const xs[1] = {x};
     float r[1];
     mlp16<1>(net, xs, r);
@@ -579,8 +647,9 @@ __global__ __launch_bounds__(kThreads) void mlp_eval_kernel(
 // ---------------------------------------------------------------------------
 // training: loss and gradient of the mean squared error over a batch
 //   loss = mean_i (mlp(u_i) - t_i)^2
-// Forward in fp32 on bf16-rounded weights, inputs and activations (the
-// precision of the MFMA path); backward in fp32 through the rounded values.
+// Forward in fp32 on bf16-rounded weights, inputs and activations, fp32
+// biases and the fp16 gate (the arithmetic of the MFMA path); backward in
+// fp32 through the rounded values.
 // Each workgroup of 64 rows writes its partial gradient and loss to a
 // workspace row; reduce_partials then sums the rows in block order, so the
 // result does not depend on the order in which workgroups finish.
@@ -618,7 +687,7 @@ __global__ __launch_bounds__(kTrainThreads) void mlp_grad_kernel(
   }
   float z1[kHidden], h1[kHidden];
   for (int h = 0; h < kHidden; ++h) {
-    float acc = round_bf16(weights[kB1 + h]);
+    float acc = weights[kB1 + h];
     for (int k = 0; k < kInputs; ++k)
       acc += round_bf16(weights[kW1 + h * kInputs + k]) * u[k];
     z1[h] = acc;
@@ -872,18 +941,20 @@ int learned_run(int32_t *data, int32_t ndim, int64_t const *extents,
     int64_t const tiles = (p.A * p.B + kTile - 1) / kTile;
     size_t const bytes = size_t(kTile) * p.m * sizeof(int32_t);
     bool const strided = p.B > 1;
-    const void *kern = strided ? reinterpret_cast<const void *>(learned_sweep<true>)
-                               : reinterpret_cast<const void *>(learned_sweep<false>);
+    bool const detrended = feature_kind == kFeaturesDetrended;
+    void (*kern)(LearnedArgs) =
+        strided ? (detrended ? learned_sweep<true, kFeaturesDetrended>
+                             : learned_sweep<true, kFeaturesRaw>)
+                : (detrended ? learned_sweep<false, kFeaturesDetrended>
+                             : learned_sweep<false, kFeaturesRaw>);
     if (bytes > 65536) {
-      rc = hip_status(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+      rc = hip_status(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
                                           static_cast<int>(bytes)));
       if (rc != PALIWA_OK)
         return rc;
     }
-    if (strided)
-      hipLaunchKernelGGL(learned_sweep<true>, dim3(uint32_t(tiles)), dim3(kThreads), bytes, stream, a);
-    else
-      hipLaunchKernelGGL(learned_sweep<false>, dim3(uint32_t(tiles)), dim3(kThreads), bytes, stream, a);
+    hipLaunchKernelGGL(kern, dim3(uint32_t(tiles)), dim3(kThreads), bytes, stream, a);
     if ((rc = hip_status(hipGetLastError())) != PALIWA_OK)
       return rc;
   }

@@ -1,8 +1,9 @@
 """numpy checker of the learned-lifting kernels (test infrastructure).
 
 * mlp / mlp_grad: the network on bf16-rounded weights, inputs and
-  activations with fp64 arithmetic, and its backpropagation (straight through
-  the roundings), to check the GPU's MFMA evaluation and gradient kernel.
+  activations, fp32 biases and the fp16 gate, with fp64 arithmetic
+  elsewhere, and its backpropagation (straight through the roundings), to
+  check the GPU's MFMA evaluation and gradient kernel.
 * integer_lifting: the integer interpolet (+ CDF 5/3 update) transform along
   every axis, restated with numpy integer arithmetic, to check the learned
   transform's fallback path bit for bit.
@@ -27,13 +28,15 @@ def bf16(x):
     return (r.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
-# The kernels' GELU: z * g(t), t = clip(z, -c, c), g = clip(0.5 + t P(t^2), 0, 1)
-# (paliwa_amd/csrc/learned.hip), a fit to the tanh form; test_learned_ref.py
-# pins it against that form and its derivative against differences.
-GELU_CLAMP = np.float32(3.4513)
-GELU_P = np.array([3.93313942e-01, -5.90118521e-02, 6.50023659e-03, -3.92342173e-04,
-                   9.70034432e-06], dtype=np.float32).astype(np.float64)
-GELU_CLAMP64 = float(GELU_CLAMP)
+# The kernels' GELU (paliwa_amd/csrc/learned.hip): z * g(z),
+#   g = clip(0.5 + s Q(s^2 - 3/8), 0, 1),  s = fp16(z / 4),
+# every step an fp16 multiply-add with one rounding; test_learned_ref.py pins
+# it against the tanh form, its exact-arithmetic version against the same
+# bound, and the derivative against differences.
+GATE_SCALE = 0.25
+GATE_SHIFT = 0.375
+GATE_Q = np.array([0.8046875, -0.9580078125, 1.541015625, -2.6484375, 2.87109375])
+GATE_SATURATION = 3.378  # |z| beyond which g is exactly 0 or 1
 
 
 def gelu_tanh(z):
@@ -41,26 +44,53 @@ def gelu_tanh(z):
     return 0.5 * z * (1.0 + np.tanh(0.7978845608028654 * (z + 0.044715 * z ** 3)))
 
 
-def _gelu_parts(z):
+def _f16(x):
+    return np.asarray(x, dtype=np.float64).astype(np.float16)
+
+
+def _fma16(a, b, c):
+    """fp16 fused multiply-add: the product of two fp16 values is exact in
+    fp64, and so is its sum with an fp16 addend over these ranges; one
+    rounding to fp16."""
+    return _f16(np.asarray(a, np.float64) * np.asarray(b, np.float64) + np.asarray(c, np.float64))
+
+
+def gate16(z):
+    """g(z) as the kernels evaluate it (fp16 steps), from fp32 or fp64 z."""
+    with np.errstate(over="ignore", invalid="ignore"):
+        s = _f16(np.asarray(z, dtype=np.float64) * GATE_SCALE)
+        w = _fma16(s, s, -GATE_SHIFT)
+        p = _fma16(GATE_Q[4], w, GATE_Q[3])
+        for k in (2, 1, 0):
+            p = _fma16(p, w, GATE_Q[k])
+        g = _fma16(s, p, 0.5)
+    return np.clip(g.astype(np.float64), 0.0, 1.0)
+
+
+def _gate_parts(z):
     z = np.asarray(z, dtype=np.float64)
-    t = np.clip(z, -GELU_CLAMP64, GELU_CLAMP64)
-    u = t * t
-    p = np.polyval(GELU_P[::-1], u)
-    raw = 0.5 + t * p
-    return z, t, u, p, raw
+    s = z * GATE_SCALE
+    w = s * s - GATE_SHIFT
+    p = np.polyval(GATE_Q[::-1], w)
+    return z, s, w, p, 0.5 + s * p
 
 
-def gelu(z):
-    z, t, u, p, raw = _gelu_parts(z)
+def gelu(z, fp16=True):
+    """The kernels' activation; fp16=False: the same polynomial in exact
+    (fp64) arithmetic, a smooth function for differentiation."""
+    if fp16:
+        return np.asarray(z, dtype=np.float64) * gate16(z)
+    z, s, w, p, raw = _gate_parts(z)
     return z * np.clip(raw, 0.0, 1.0)
 
 
 def gelu_grad(z):
-    z, t, u, p, raw = _gelu_parts(z)
+    """d/dz of gelu(z, fp16=False)."""
+    z, s, w, p, raw = _gate_parts(z)
     g = np.clip(raw, 0.0, 1.0)
-    dp = np.polyval((GELU_P[1:] * np.arange(1, 5))[::-1], u)
-    dg = p + 2.0 * u * dp
-    inside = (np.abs(z) < GELU_CLAMP64) & (raw > 0.0) & (raw < 1.0)
+    dp = np.polyval((GATE_Q[1:] * np.arange(1, 5))[::-1], w)
+    dg = GATE_SCALE * (p + 2.0 * s * s * dp)
+    inside = (raw > 0.0) & (raw < 1.0)
     return np.where(inside, g + z * dg, g)
 
 
@@ -75,10 +105,10 @@ def mlp(u, w, rounding=True):
     rnd = bf16 if rounding else (lambda v: np.asarray(v, dtype=np.float64))
     w1, b1, w2, b2, w3, b3 = unpack(w)
     x = rnd(u)
-    z1 = x @ rnd(w1).T + rnd(b1)
-    h1 = rnd(gelu(z1))
+    z1 = x @ rnd(w1).T + b1.astype(np.float64)
+    h1 = rnd(gelu(z1, rounding))
     z2 = h1 @ rnd(w2).T + b2.astype(np.float64)
-    h2 = rnd(gelu(z2))
+    h2 = rnd(gelu(z2, rounding))
     return h2 @ rnd(w3) + float(b3)
 
 
@@ -89,10 +119,10 @@ def mlp_grad(u, t, w, rounding=True):
     w1, b1, w2, b2, w3, b3 = unpack(w)
     n = u.shape[0]
     x = rnd(u)
-    z1 = x @ rnd(w1).T + rnd(b1)
-    h1 = rnd(gelu(z1))
+    z1 = x @ rnd(w1).T + b1.astype(np.float64)
+    h1 = rnd(gelu(z1, rounding))
     z2 = h1 @ rnd(w2).T + b2.astype(np.float64)
-    h2 = rnd(gelu(z2))
+    h2 = rnd(gelu(z2, rounding))
     r = h2 @ rnd(w3) + float(b3)
     err = r - t
     loss = np.mean(err ** 2)
