@@ -671,12 +671,14 @@ paliwa_batch::Group single_group(paliwa_batch const &b, int32_t phase, int32_t d
     SweepParams const q = make_params(b.data[g], b.ndim, b.extents_of(g), b.level_of(g), d);
     grp.lines += q.A * q.B;
   }
-  // long lines: the long-line kernel on strided axes, the LDS-line kernel on
-  // the contiguous one as far as it holds the line (as the single-grid
-  // schedule chooses), the long-line kernel beyond
+  // long lines: the long-line kernel on strided axes and on contiguous lines
+  // its staging buffer holds, the LDS-line kernel on longer contiguous lines
+  // as far as it holds them (as the single-grid schedule chooses), the
+  // long-line kernel beyond
   int64_t const n = int64_t(1) << grp.L;
   grp.long_ladder = !grp.fused && n <= long_ladder_max_period(dtype) &&
-                    (grp.strided || n > lds_line_max_period(dtype));
+                    (grp.strided || long_contiguous_staged(dtype, grp.L) ||
+                     n > lds_line_max_period(dtype));
   grp.lds_lines = !grp.fused && !grp.long_ladder && n <= lds_line_max_period(dtype);
   if (grp.long_ladder) {
     grp.threads = long_threads_for(dtype, grp.L, grp.lines);

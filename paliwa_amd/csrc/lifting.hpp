@@ -197,6 +197,9 @@ bool long_ladder_fits(int dtype, SweepParams const &p);
 // the automatic schedule's choice: strided axes, and contiguous lines longer
 // than the LDS-line kernel holds
 bool long_ladder_preferred(int dtype, SweepParams const &p);
+// contiguous lines of period 2^L that go through the long-line kernel's LDS
+// staging buffer (a few lines per round)
+bool long_contiguous_staged(int dtype, int L);
 int long_ladder_max_period(int dtype);
 int long_threads_for(int dtype, int L, int64_t lines);
 int64_t long_tile_lines(int dtype, int L, int threads);

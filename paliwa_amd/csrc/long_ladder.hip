@@ -18,9 +18,13 @@
 //     LDS, the remaining levels run there with one barrier per pass, and they
 //     come back to the registers.
 // Along a strided axis the lanes run across the tile's lines (the TB columns
-// of a row are consecutive in memory, so each row access is TB points wide);
-// along the contiguous axis they run along the line (every lane reads its K
-// consecutive points).  Each update is c0*g[lower] + c1*g[x] + c2*g[upper],
+// of a row are consecutive in memory, so each row access is TB points wide).
+// Along the contiguous axis a line is one run of memory: the tile's lines go
+// through an LDS staging buffer, a few lines at a time, with coalesced copies
+// (consecutive threads, consecutive points), and each thread takes its chunk
+// from a layout with one pad point after every chunk (chunks K points apart
+// would put a wavefront's reads in one bank); lines too long for the buffer
+// read their chunks straight from memory.  Each update is c0*g[lower] + c1*g[x] + c2*g[upper],
 // left to right without contraction, exactly the reference's expression, and
 // a pass is a pure function of the state before it (written and read points
 // have opposite parity), so the results are bit-identical to the reference's.
@@ -35,6 +39,19 @@ namespace paliwa_hip {
 namespace {
 
 constexpr int kLongThreads = 1024; // the widest tile; 256-thread tiles for few lines
+// LDS staging of contiguous tiles: whole padded lines (n + n / K points), as
+// many per round as fit
+constexpr int kStageBytes = 68 * 1024;
+
+template <typename T> constexpr int long_chunk() { return sizeof(T) == 8 ? 32 : 64; }
+
+// Padded lines of period 2^L that one staging round holds (0: the line does
+// not fit, its chunks are read from memory).
+template <typename T> __host__ __device__ constexpr int stage_lines_of(int L) {
+  int64_t const padded = ((int64_t(1) << L) + ((int64_t(1) << L) / long_chunk<T>())) *
+                         int64_t(sizeof(T));
+  return static_cast<int>(kStageBytes / padded);
+}
 
 template <typename T> struct LongCoefs {
   T c[2][3];
@@ -94,6 +111,8 @@ __device__ __forceinline__ void long_tiles(SweepParams const &p, Src const &src,
                                            int64_t ntiles) {
   __shared__ T xch[2][NT];
   __shared__ T heads[NT];
+  // staging buffer of contiguous tiles (one element in strided kernels)
+  __shared__ T stage[STRIDED ? 1 : kStageBytes / sizeof(T)];
   constexpr int LOGK = fused::ilog2(K);
   int const tid = threadIdx.x;
   int const n = 1 << p.L;
@@ -122,15 +141,48 @@ __device__ __forceinline__ void long_tiles(SweepParams const &p, Src const &src,
       step = ref.B;
     }
     T v[K];
-    if (active) {
-      T const *q = base + int64_t(geo.gi) * K * step;
+    // contiguous tiles through the staging buffer: rounds of R lines, line r
+    // of a round at stage[r * P], chunk g of it at g * (K + 1)
+    [[maybe_unused]] int const R = STRIDED ? 0 : min(TB, stage_lines_of<T>(p.L));
+    [[maybe_unused]] int const P = n + G;
+    [[maybe_unused]] int64_t const first = ref.local * TB; // the tile's first line
+    [[maybe_unused]] int64_t const nlines = ref.A * ref.B;
+    [[maybe_unused]] T *const lines0 = static_cast<T *>(ref.data) + first * int64_t(m);
+    if constexpr (!STRIDED) {
+      if (R > 0) {
 #pragma unroll
-      for (int j = 0; j < K; ++j)
-        v[j] = q[j * step];
-    } else {
+        for (int j = 0; j < K; ++j) // lines past the grid's end stay 0
+          v[j] = T(0);
+        for (int r0 = 0; r0 < TB; r0 += R) {
+          for (int r = 0; r < R && r0 + r < TB; ++r) {
+            if (first + r0 + r >= nlines)
+              break;
+            T const *src = lines0 + int64_t(r0 + r) * m;
+            for (int x = tid; x < n; x += NT)
+              stage[r * P + x + (x >> LOGK)] = src[x];
+          }
+          __syncthreads();
+          if (geo.li >= r0 && geo.li < r0 + R) {
+            T const *c = stage + (geo.li - r0) * P + geo.gi * (K + 1);
 #pragma unroll
-      for (int j = 0; j < K; ++j)
-        v[j] = T(0);
+            for (int j = 0; j < K; ++j)
+              v[j] = c[j];
+          }
+          __syncthreads(); // the next round overwrites the buffer
+        }
+      }
+    }
+    if (STRIDED || R == 0) {
+      if (active) {
+        T const *q = base + int64_t(geo.gi) * K * step;
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+          v[j] = q[j * step];
+      } else {
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+          v[j] = T(0);
+      }
     }
 
     // fine levels (stride 2^e < K) in registers
@@ -197,6 +249,30 @@ __device__ __forceinline__ void long_tiles(SweepParams const &p, Src const &src,
       }(std::make_integer_sequence<int, LOGK>{});
     }
 
+    if constexpr (!STRIDED) {
+      if (R > 0) {
+        for (int r0 = 0; r0 < TB; r0 += R) {
+          if (geo.li >= r0 && geo.li < r0 + R) {
+            T *c = stage + (geo.li - r0) * P + geo.gi * (K + 1);
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+              c[j] = v[j];
+          }
+          __syncthreads();
+          for (int r = 0; r < R && r0 + r < TB; ++r) {
+            if (first + r0 + r >= nlines)
+              break;
+            T *dst = lines0 + int64_t(r0 + r) * m;
+            for (int x = tid; x < n; x += NT)
+              dst[x] = stage[r * P + x + (x >> LOGK)];
+            if (m != n && tid == 0) // periodic image: a copy of index 0
+              dst[n] = stage[r * P];
+          }
+          __syncthreads(); // the next round (or tile) overwrites the buffer
+        }
+        continue;
+      }
+    }
     if (active) {
       T *q = base + int64_t(geo.gi) * K * step;
 #pragma unroll
@@ -218,8 +294,6 @@ __global__ __launch_bounds__(NT) void long_ladder_batched(SweepParams p, int64_t
                                                           ManyGrids grids) {
   long_tiles<T, K, NT, STRIDED, PAT, ASC>(p, grids, ntiles);
 }
-
-template <typename T> constexpr int long_chunk() { return sizeof(T) == 8 ? 32 : 64; }
 
 template <int NT, typename Kernel, typename... Extra>
 hipError_t launch_long_kernel(Kernel kernel, SweepParams const &p, int64_t tiles,
@@ -295,12 +369,13 @@ bool long_ladder_fits(int dtype, SweepParams const &p) {
 }
 
 bool long_ladder_preferred(int dtype, SweepParams const &p) {
-  // along the contiguous axis a lane reads its own K consecutive points, which
-  // no load instruction coalesces: the LDS-line kernel, which stages a whole
-  // line with coalesced copies, is faster there (4097^2 float64: 110 against
-  // 131 us) for as long as it holds the line
   return long_ladder_fits(dtype, p) &&
-         (p.B > 1 || (int64_t(1) << p.L) > lds_line_max_period(dtype));
+         (p.B > 1 || long_contiguous_staged(dtype, p.L) ||
+          (int64_t(1) << p.L) > lds_line_max_period(dtype));
+}
+
+bool long_contiguous_staged(int dtype, int L) {
+  return (dtype == PALIWA_FLOAT64 ? stage_lines_of<double>(L) : stage_lines_of<float>(L)) > 0;
 }
 
 int long_ladder_max_period(int dtype) {

@@ -297,6 +297,29 @@ def test_long_ladder_wide_tiles(pw, dtype, L):
         assert_same(back, oracle_named(fwd, level, lmin, name, False), f"{extents} {name} inv")
 
 
+@pytest.mark.parametrize("dtype,L", [(np.float64, 12), (np.float64, 13), (np.float32, 13),
+                                     (np.float32, 14)])
+def test_long_ladder_contiguous_staged(pw, dtype, L):
+    """Contiguous lines the long-line kernel stages through LDS (float64 up
+    to 2^13, float32 up to 2^14 points), a few lines per round: 2049 lines
+    (1024-thread tiles, several rounds per tile, a partial last tile) and 5
+    lines (256-thread tiles), with and without images, both directions."""
+    rng = np.random.default_rng(21)
+    n = 1 << L
+    for lines, image in ((2049, True), (2049, False), (5, True)):
+        extents = [lines, n + 1 if image else n]
+        level = [11 if lines == 2049 else 2, L]
+        lmin = [level[0], 0]
+        for name in ("hat", "biorthogonal"):
+            a = closed_random(rng, extents, level, dtype)
+            fwd = gpu_named(pw, a, level, lmin, name, True)
+            assert_same(fwd, oracle_named(a, level, lmin, name, True),
+                        f"{extents} {name} fwd")
+            back = gpu_named(pw, fwd, level, lmin, name, False)
+            assert_same(back, oracle_named(fwd, level, lmin, name, False),
+                        f"{extents} {name} inv")
+
+
 def test_long_ladder_rejects_generic_lists(pw):
     """The long-line kernel runs ladders only: an explicit request with a
     non-consecutive level list is refused; the automatic schedule takes the
