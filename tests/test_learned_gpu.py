@@ -36,9 +36,16 @@ def test_mfma_network_matches_fp64(ll):
     want = lr.mlp(u, w)
     diff = np.abs(got - want)
     scale = np.abs(want).max()
-    # fp32 accumulation against fp64; a bf16 rounding of a hidden activation
-    # may fall the other way for a few points
+    # fp32 accumulation against fp64: almost every point agrees to 1e-5
     assert np.quantile(diff, 0.99) <= 1e-5 * scale, np.quantile(diff, 0.99)
+    # Where a pre-activation lies at an fp16 or bf16 rounding boundary, the
+    # fp32 and fp64 values may round to neighbours and one activation differs
+    # by a bf16 step (2^-7 of its magnitude).  On these weights one such flip
+    # of the most influential unit (2^-7 |h| |w3|, or for a layer-1 unit
+    # through |W2|, |w3| and the slope <= 1.13) moves the output by up to
+    # 0.028 scale at the worst point, 0.012 at the median one: the bound below
+    # admits one flip of a typical unit, and such points must stay rare.
+    assert np.mean(diff > 1e-5 * scale) <= 0.01, np.mean(diff > 1e-5 * scale)
     assert diff.max() <= 2e-2 * scale, diff.max()
 
 
