@@ -27,7 +27,9 @@
 //   * the execution space is paliwa::HipExec, a HIP stream (the reference's
 //     ExecSpace instance), passed explicitly as the reference's callers do
 //     (its ExecSpace template parameter has no default); calls are
-//     asynchronous on it and never synchronize.
+//     asynchronous on it and never synchronize, except the first call of the
+//     partial-output transform_in with a given set of arguments, which
+//     creates its plan (below).
 // Errors: an unknown wavelet name throws std::out_of_range (std::map::at in
 // the reference), a filter offset other than 0/1 throws
 // std::runtime_error("Filter offset not supported"), violated level ranges
