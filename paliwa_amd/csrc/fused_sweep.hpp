@@ -104,27 +104,69 @@ __device__ __forceinline__ int32_t lift(int32_t sign, int32_t lo, int32_t bias,
   return static_cast<int32_t>(out);
 }
 
+// Cross-lane moves within a line's group of G consecutive lanes (aligned to
+// G).  Groups of 2 and 4 lanes use DPP quad permutations, groups of 8 and 16
+// DPP row rotations (row_ror:k gives lane i the value of lane (i - k) mod 16
+// of its 16-lane row); groups of 32 and 64 lanes, which no single DPP move
+// rotates, use ds_bpermute.  A DPP move is a VALU instruction per 32-bit word.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_word(uint32_t x) {
+  return static_cast<uint32_t>(
+      __builtin_amdgcn_mov_dpp(static_cast<int>(x), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_move(T v) {
+  if constexpr (sizeof(T) == 8) {
+    uint64_t const b = __builtin_bit_cast(uint64_t, v);
+    uint32_t const lo = dpp_word<CTRL>(static_cast<uint32_t>(b));
+    uint32_t const hi = dpp_word<CTRL>(static_cast<uint32_t>(b >> 32));
+    return __builtin_bit_cast(T, (uint64_t(hi) << 32) | lo);
+  } else {
+    return __builtin_bit_cast(T, dpp_word<CTRL>(__builtin_bit_cast(uint32_t, v)));
+  }
+}
+constexpr int dpp_row_ror(int k) { return 0x120 + k; } // k in 1..15
+// quad_perm control: lane j of a quad takes lane perm[j], the lane d places
+// further in j's group of G (G = 2 or 4)
+template <int G, int D> constexpr int dpp_quad_rotate() {
+  int c = 0;
+  for (int j = 0; j < 4; ++j)
+    c |= ((j & ~(G - 1)) + (((j % G) + D) % G)) << (2 * j);
+  return c;
+}
+
+// The value v of lane gbase + ((gi + D) mod G), -G < D < G.
+template <typename T, int G, int D>
+__device__ __forceinline__ T group_fetch(T v, int gbase, int gi) {
+  constexpr int d = ((D % G) + G) % G;
+  if constexpr (d == 0) {
+    return v;
+  } else if constexpr (G <= 4) {
+    return dpp_move<dpp_quad_rotate<G, d>()>(v);
+  } else if constexpr (G == 16) {
+    return dpp_move<dpp_row_ror(16 - d)>(v);
+  } else if constexpr (G == 8) { // two rotations of the row, one per side of the group end
+    T const ahead = dpp_move<dpp_row_ror(16 - d)>(v); // lane i + d
+    T const wrap = dpp_move<dpp_row_ror(8 - d)>(v);   // lane i + d - 8
+    return gi + d < 8 ? ahead : wrap;
+  } else {
+    return __shfl(v, gbase + ((gi + d) & (G - 1)), 64);
+  }
+}
+
 // One pass at storage stride S < K on this lane's chunk.
 template <typename T, int K, int G, int S, int ODD>
 __device__ __forceinline__ void pass_in_lane(T (&v)[K], int gbase, int gi,
                                              T c0, T c1, T c2) {
   if constexpr (ODD) {
-    T nb;
-    if constexpr (G == 1)
-      nb = v[0];
-    else
-      nb = __shfl(v[0], gbase + ((gi + 1) & (G - 1)), 64);
+    T const nb = group_fetch<T, G, 1>(v[0], gbase, gi);
 #pragma unroll
     for (int j = S; j < K; j += 2 * S) {
       T const hi = (j + S < K) ? v[j + S] : nb;
       v[j] = lift(c0, v[j - S], c1, v[j], c2, hi);
     }
   } else {
-    T pb;
-    if constexpr (G == 1)
-      pb = v[K - S];
-    else
-      pb = __shfl(v[K - S], gbase + ((gi - 1) & (G - 1)), 64);
+    T const pb = group_fetch<T, G, -1>(v[K - S], gbase, gi);
 #pragma unroll
     for (int j = 0; j < K; j += 2 * S) {
       T const lo = (j >= S) ? v[j - S] : pb;
@@ -133,13 +175,30 @@ __device__ __forceinline__ void pass_in_lane(T (&v)[K], int gbase, int gi,
   }
 }
 
-// One pass at a stride that is q >= 1 chunks long.
+// One pass at a stride that is q >= 1 chunks long (q a power of two below G).
 template <typename T, int K, int G>
 __device__ __forceinline__ void pass_cross_lane(T (&v)[K], int gbase, int gi,
                                                 int q, int odd, T c0, T c1,
                                                 T c2) {
-  T const lo = __shfl(v[0], gbase + ((gi - q) & (G - 1)), 64);
-  T const hi = __shfl(v[0], gbase + ((gi + q) & (G - 1)), 64);
+  T lo, hi;
+  if constexpr (G <= 16) {
+    auto fetch = [&](auto q_tag) {
+      constexpr int Q = decltype(q_tag)::value;
+      lo = group_fetch<T, G, -Q>(v[0], gbase, gi);
+      hi = group_fetch<T, G, Q>(v[0], gbase, gi);
+    };
+    if (q == 1)
+      fetch(std::integral_constant<int, 1>{});
+    else if (G > 4 && q == 4)
+      fetch(std::integral_constant<int, (G > 4 ? 4 : 1)>{});
+    else if (G > 8 && q == 8)
+      fetch(std::integral_constant<int, (G > 8 ? 8 : 1)>{});
+    else
+      fetch(std::integral_constant<int, (G > 2 ? 2 : 1)>{});
+  } else {
+    lo = __shfl(v[0], gbase + ((gi - q) & (G - 1)), 64);
+    hi = __shfl(v[0], gbase + ((gi + q) & (G - 1)), 64);
+  }
   if ((gi & (2 * q - 1)) == odd * q)
     v[0] = lift(c0, lo, c1, v[0], c2, hi);
 }
