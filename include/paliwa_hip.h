@@ -64,7 +64,10 @@ extern "C" {
                                     returns PALIWA_ERR_UNSUPPORTED */
 
 /* Kernel schedules for paliwa_transform_in_path. */
-#define PALIWA_PATH_AUTO 0     /* best available (fused when it fits) */
+#define PALIWA_PATH_AUTO 0     /* fused where it holds the line, else the
+                                  long-line kernel (strided axes, contiguous
+                                  lines its staging buffer holds), else the
+                                  LDS-line kernel, else per pass */
 #define PALIWA_PATH_FUSED 1    /* one launch per axis: lines staged in LDS,
                                   every level and pass in registers */
 #define PALIWA_PATH_LDS_LINE 2 /* one launch per axis: one line per workgroup
