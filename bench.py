@@ -604,8 +604,9 @@ def learned_benchmark():
         "forward_Gpoints_per_s": round(pts / (le["forward_ms"] * 1e-3) / 1e9, 2),
         # the network at every level >= 3 of every axis: its arithmetic against
         # the bf16 dense MFMA peak (2 * (8*32 + 32*32 + 32) = 2624 flop per
-        # evaluation); the sweep is bound by the VALU (two quarter-rate
-        # transcendentals per hidden unit in the GELU), not by the matrix cores
+        # evaluation); the sweep is bound by the VALU (the packed-fp16 GELU
+        # gate and the bf16 conversions of the activations), not by the matrix
+        # cores
         "network": network_rate(r["learned_without_fallback"]),
         "train": {k: r["train"][k] for k in ("pairs", "steps", "batch", "loss_before",
                                              "loss_after", "seconds")},

@@ -763,8 +763,19 @@ __global__ void reduce_partials(float const *partials, int32_t blocks,
   int const e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= kPartial)
     return;
+  // block order, 16 loads in flight at a time (the adds stay in that order)
   float acc = 0.0f;
-  for (int b = 0; b < blocks; ++b)
+  int b = 0;
+  for (; b + 16 <= blocks; b += 16) {
+    float v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      v[j] = partials[int64_t(b + j) * kPartial + e];
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      acc += v[j];
+  }
+  for (; b < blocks; ++b)
     acc += partials[int64_t(b) * kPartial + e];
   if (e < kParams)
     grad[e] += acc;

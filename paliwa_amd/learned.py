@@ -257,8 +257,8 @@ def detail_stats(coeffs: torch.Tensor, level, lmin, threshold: float):
 
 
 def workload(level=(9, 9, 9), train_level=(7, 7, 7), train_seeds=range(100, 132),
-             test_seed=4, validation_seed=200, pairs_count=1 << 22, steps=3000,
-             batch=4096, lr=2e-3, update=True, features=FEATURES_DETRENDED, device="cuda",
+             test_seed=4, validation_seed=200, pairs_count=1 << 22, steps=10000,
+             batch=16384, lr=2e-3, update=True, features=FEATURES_DETRENDED, device="cuda",
              log=None):
     """The learned-lifting workload: train on turbulent-like fields (64
     Fourier modes with a k^-5/3 spectrum plus 5 tanh fronts of width 0.01),
@@ -269,7 +269,10 @@ def workload(level=(9, 9, 9), train_level=(7, 7, 7), train_seeds=range(100, 132)
     above 1e-4 max|f|, detail L1, and the forward+inverse round trip.  Also
     reports masks probed on the test field itself (selection on the test
     data, labelled as such) and the float interpolet of the plain library on
-    the same field."""
+    the same field.  The training length and minibatch (10000 steps of 16384
+    pairs) were chosen on the validation field's detail L1 ratio, over three
+    training seeds each: 0.861-0.903, against 0.923-0.936 for 3000 steps of
+    4096, 0.900-0.926 for 20000 of 8192 and 0.877-0.914 for 30000 of 16384."""
     import time
 
     from . import fields
