@@ -528,7 +528,7 @@ int paliwa_learned_extract_pairs(int32_t *data, int32_t ndim,
 int paliwa_learned_mlp_eval(const float *inputs, const float *weights,
                             float *out, int32_t count, void *stream);
 
-/* Adds to grad the gradient of mean_i (mlp(u_i) - t_i)^2 over rows
+/* Adds to grad the gradient of mean_i |mlp(u_i) - t_i| over rows
  * index[0..count) (or 0..count) of pairs (row stride `stride` floats: 8
  * inputs then the target), and the loss to *loss_sum.  `workspace`: device
  * floats, paliwa_learned_gradient_workspace(count) of them, for per-workgroup

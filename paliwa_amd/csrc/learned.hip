@@ -45,10 +45,7 @@ constexpr int kLogTile = 4;
 constexpr int kHidden = 32;
 constexpr int kInputs = 8;
 // network groups of 16 points interleaved per wavefront in the transform
-#ifndef PALIWA_LEARNED_GROUPS
-#define PALIWA_LEARNED_GROUPS 2
-#endif
-constexpr int kGroupsInFlight = PALIWA_LEARNED_GROUPS;
+constexpr int kGroupsInFlight = 2;
 // Packed fp32 parameter layout.
 constexpr int kW1 = 0;                         // [32][8]
 constexpr int kB1 = kW1 + kHidden * kInputs;   // [32]
@@ -645,8 +642,12 @@ __global__ __launch_bounds__(kThreads) void mlp_eval_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// training: loss and gradient of the mean squared error over a batch
-//   loss = mean_i (mlp(u_i) - t_i)^2
+// training: loss and gradient of the mean absolute error over a batch
+//   loss = mean_i |mlp(u_i) - t_i|,  d loss / d r_i = sign(r_i - t_i) / n
+// (0 where r_i == t_i).  The absolute error is the detail L1 the predict is
+// meant to lower; its minimizer is the conditional median of the target,
+// which the heavy tails of the residuals near fronts do not pull the way
+// they pull the mean a squared error fits.
 // Forward in fp32 on bf16-rounded weights, inputs and activations, fp32
 // biases and the fp16 gate (the arithmetic of the MFMA path); backward in
 // fp32 through the rounded values.
@@ -706,8 +707,9 @@ __global__ __launch_bounds__(kTrainThreads) void mlp_grad_kernel(
     r += round_bf16(weights[kW3 + h]) * h2;
   }
   float const err = valid ? r - t : 0.0f;
-  float const d3 = 2.0f * err * scale; // d loss / d r, scale = 1/batch
-  sh_loss[tid] = err * err * scale;
+  // d loss / d r, scale = 1/batch
+  float const d3 = (err > 0.0f ? 1.0f : (err < 0.0f ? -1.0f : 0.0f)) * scale;
+  sh_loss[tid] = fabsf(err) * scale;
   sh_d3[tid] = d3;
   for (int h = 0; h < kHidden; ++h) {
     float const d2 = d3 * round_bf16(weights[kW3 + h]) * gelu_grad(z2[h]);

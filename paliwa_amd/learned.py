@@ -119,7 +119,7 @@ def _workspace(count: int, device) -> torch.Tensor:
 
 
 def gradient(pairs: torch.Tensor, weights: torch.Tensor, index: torch.Tensor | None = None):
-    """(loss, gradient) of mean (mlp(u) - t)^2 over the rows of pairs [n, >= 9]
+    """(loss, gradient) of mean |mlp(u) - t| over the rows of pairs [n, >= 9]
     (deterministic: partial sums are added in a fixed order)."""
     pairs = pairs.contiguous()
     grad = torch.zeros(PARAMS, dtype=torch.float32, device=pairs.device)
@@ -178,8 +178,9 @@ def extract_pairs(grids: Sequence[torch.Tensor], level, lmin, capacity: int,
 
 def train(pairs: torch.Tensor, steps: int = 3000, batch: int = 4096, lr: float = 2e-3,
           seed: int = 0, weights: torch.Tensor | None = None, log_every: int = 0):
-    """Adam on the mean squared normalized residual, all in the HIP kernels
-    (gradient, Adam); minibatches drawn with a seeded generator."""
+    """Adam on the mean absolute normalized residual (the detail L1 the
+    predict is meant to lower), all in the HIP kernels (gradient, Adam);
+    minibatches drawn with a seeded generator."""
     device = pairs.device
     w = init_weights(seed, device) if weights is None else weights.clone()
     m1 = torch.zeros(PARAMS, dtype=torch.float32, device=device)
@@ -269,10 +270,11 @@ def workload(level=(9, 9, 9), train_level=(7, 7, 7), train_seeds=range(100, 132)
     above 1e-4 max|f|, detail L1, and the forward+inverse round trip.  Also
     reports masks probed on the test field itself (selection on the test
     data, labelled as such) and the float interpolet of the plain library on
-    the same field.  The training length and minibatch (10000 steps of 16384
-    pairs) were chosen on the validation field's detail L1 ratio, over three
-    training seeds each: 0.861-0.903, against 0.923-0.936 for 3000 steps of
-    4096, 0.900-0.926 for 20000 of 8192 and 0.877-0.914 for 30000 of 16384."""
+    the same field.  The training loss (the mean absolute residual), length
+    and minibatch (10000 steps of 16384 pairs) were chosen on the validation
+    field's detail L1 ratio over three training seeds each: 0.602-0.619,
+    against 0.861-0.903 with the mean squared residual and 0.605-0.620 /
+    0.614-0.630 for 3000 steps of 4096 / 30000 of 16384."""
     import time
 
     from . import fields

@@ -113,8 +113,8 @@ def mlp(u, w, rounding=True):
 
 
 def mlp_grad(u, t, w, rounding=True):
-    """(loss, gradient) of mean (mlp(u) - t)^2, fp64; with rounding, straight
-    through the bf16 roundings."""
+    """(loss, gradient) of mean |mlp(u) - t|, fp64; with rounding, straight
+    through the bf16 roundings (d loss / d r = sign(r - t) / n)."""
     rnd = bf16 if rounding else (lambda v: np.asarray(v, dtype=np.float64))
     w1, b1, w2, b2, w3, b3 = unpack(w)
     n = u.shape[0]
@@ -125,8 +125,8 @@ def mlp_grad(u, t, w, rounding=True):
     h2 = rnd(gelu(z2, rounding))
     r = h2 @ rnd(w3) + float(b3)
     err = r - t
-    loss = np.mean(err ** 2)
-    d3 = 2.0 * err / n
+    loss = np.mean(np.abs(err))
+    d3 = np.sign(err) / n
     g = np.zeros(P)
     g[W3:B3] = h2.T @ d3
     g[B3] = d3.sum()
