@@ -176,7 +176,7 @@ def extract_pairs(grids: Sequence[torch.Tensor], level, lmin, capacity: int,
     return pairs[order][:capacity].contiguous()
 
 
-def train(pairs: torch.Tensor, steps: int = 3000, batch: int = 4096, lr: float = 2e-3,
+def train(pairs: torch.Tensor, steps: int = 10000, batch: int = 16384, lr: float = 2e-3,
           seed: int = 0, weights: torch.Tensor | None = None, log_every: int = 0):
     """Adam on the mean absolute normalized residual (the detail L1 the
     predict is meant to lower), all in the HIP kernels (gradient, Adam);
