@@ -66,6 +66,9 @@ def parse():
                          "of the Infinity Cache and after a 512 MB read flush (counter runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
+    ap.add_argument("--pitch-probe", action="store_true",
+                    help="only time the strided axis-0 sweep of 257 x X x Y float64 grids "
+                         "of about the 257^3 grid's size at several row pitches")
     return ap.parse_args()
 
 
@@ -120,6 +123,57 @@ def cold_probe(torch, reps=5):
             key = f"{'fwd' if fwd else 'inv'}_dim{d}"
             out[key] = {k: round(sorted(v)[len(v) // 2], 4) for k, v in times.items()}
     print(json.dumps({"cold_probe_ms": out, "bytes_per_sweep": 2 * grid.numel() * 8}))
+
+
+def pitch_probe(torch, reps=50):
+    """The strided sweep of axis 0 (lines of 257 points, biorthogonal,
+    forward and inverse alternating, HIP events over `reps` of each) on
+    257 x X x Y float64 grids with X * Y within 6% of 257^2: the same kernel,
+    tile shape and bytes per line, rows X * Y * 8 bytes apart.  A row of a
+    32-column tile is 256 bytes; it starts on a 128-byte line boundary in
+    every row only when the pitch is a multiple of 16 points."""
+    import ctypes
+    import math
+
+    from paliwa_amd import _lib
+
+    stream = torch.cuda.current_stream()
+
+    def level_of(x):
+        return int(math.log2(x)) if x & (x - 1) == 0 else int(math.log2(x - 1))
+
+    out = []
+    shapes = ((256, 256), (2049, 32), (1025, 64), (513, 128), (512, 129), (257, 256),
+              (257, 257), (129, 513), (65, 1025), (33, 2049))
+    # the list twice, the first pass untimed in the report (clocks settle)
+    for k, (X, Y) in enumerate(shapes + shapes):
+        shape = (257, X, Y)
+        level = [8, level_of(X), level_of(Y)]
+        g = torch.randn(shape, dtype=torch.float64, device="cuda")
+        args = (ctypes.c_void_p(g.data_ptr()), _lib.PALIWA_FLOAT64, 3, _lib.i64_array(shape),
+                _lib.i32_array(level), _lib.i32_array([0] * 3), _lib.i32_array(level), 0,
+                b"biorthogonal", ctypes.c_void_p(stream.cuda_stream))
+        fns = (_lib.lib.paliwa_hierarchize_in, _lib.lib.paliwa_dehierarchize_in)
+        for _ in range(3):
+            for fn in fns:
+                assert fn(*args) == 0, _lib.last_error()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            for fn in fns:
+                fn(*args)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) / (2 * reps) * 1e3
+        pitch = X * Y * 8
+        if k >= len(shapes):
+            out.append({"shape": list(shape), "pitch_bytes": pitch,
+                        "pitch_mod_128": pitch % 128, "us": round(us, 2),
+                        "GBps": round(2 * g.numel() * 8 / (us * 1e-6) / 1e9, 1)})
+        del g
+    print(json.dumps({"pitch_probe": out}))
 
 
 def partial_benchmark(torch, ranks=8, reps=20):
@@ -640,6 +694,9 @@ def main():
         return
     if args.cold_probe:
         cold_probe(torch)
+        return
+    if args.pitch_probe:
+        pitch_probe(torch)
         return
 
     import paliwa_amd as pw
