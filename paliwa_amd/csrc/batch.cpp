@@ -1075,11 +1075,14 @@ int tune(paliwa_batch *b, char const *wavelet_name, bool forward, char const *ca
   }
   if (rc == PALIWA_OK && timing)
     rc = evaluate(&best);
-  // coordinate descent: one choice at a time, kept when at least 1% faster
-  std::sort(search.begin(), search.end(), [&](size_t x, size_t y) {
-    return b->groups[x].lines * (int64_t(1) << b->groups[x].L) >
-           b->groups[y].lines * (int64_t(1) << b->groups[y].L);
-  });
+  // coordinate descent: one choice at a time, kept when at least 1% faster.
+  // `search` indexes the groups of the chosen box schedule; after a failed
+  // step the plan may hold another candidate's groups, so it is not touched.
+  if (rc == PALIWA_OK)
+    std::sort(search.begin(), search.end(), [&](size_t x, size_t y) {
+      return b->groups[x].lines * (int64_t(1) << b->groups[x].L) >
+             b->groups[y].lines * (int64_t(1) << b->groups[y].L);
+    });
   for (int round = 0; round < 2 && rc == PALIWA_OK; ++round) {
     for (size_t i : search) {
       auto &grp = b->groups[i];
