@@ -144,7 +144,9 @@ int sweep(int32_t dtype, SweepParams p, int32_t path, hipStream_t stream) {
   paliwa_hip::classify(p);
   int64_t const n = int64_t(1) << p.L;
   if (path == PALIWA_PATH_AUTO) {
-    if (n <= paliwa_hip::fused_max_period(dtype))
+    if (paliwa_hip::long_columns_preferred(dtype, p))
+      path = PALIWA_PATH_LONG_LADDER; // column tiles
+    else if (n <= paliwa_hip::fused_max_period(dtype))
       path = PALIWA_PATH_FUSED;
     else if (paliwa_hip::long_ladder_preferred(dtype, p))
       path = PALIWA_PATH_LONG_LADDER;
@@ -172,7 +174,8 @@ int sweep(int32_t dtype, SweepParams p, int32_t path, hipStream_t stream) {
     if (!paliwa_hip::long_ladder_fits(dtype, p))
       return fail(PALIWA_ERR_UNSUPPORTED,
                   "the long-line kernel takes lines above the fused kernel's periods "
-                  "and consecutive level lists with the named tables' parities");
+                  "(and strided float64 lines of 256 points) and consecutive level "
+                  "lists with the named tables' parities");
     err = paliwa_hip::launch_long_ladder(dtype, p, stream);
     break;
   default:

@@ -193,10 +193,21 @@ hipError_t launch_per_pass(int dtype, SweepParams const &p,
 // long_ladder_max_period, ladder schedules only; tiles of 256 or 1024 threads
 // (long_threads_for: 256 where 1024-thread tiles would be fewer than the CUs
 // and a line fits 256 threads), long_tile_lines(dtype, L, threads) lines each.
+// Also the column tiles: strided float64 lines of 2^8 points in chunks of 16
+// (64 columns per 1024-thread tile, two tiles per CU).
 bool long_ladder_fits(int dtype, SweepParams const &p);
 // the automatic schedule's choice: strided axes, and contiguous lines longer
 // than the LDS-line kernel holds
 bool long_ladder_preferred(int dtype, SweepParams const &p);
+// the automatic schedule's choice of the column tiles over the fused kernel:
+// filter tables of two passes per level, at least 16384 lines (1024-thread
+// tiles), and either at least 4096 lines between consecutive points (an outer
+// axis) or a grid of at most 160 MiB (Infinity-Cache-resident when swept
+// repeatedly); measured against the fused kernel (biorthogonal): 257^3 axes 0
+// and 1 -5 to -6%, 257 x 1025^2 axis 0 -4%, 257^2 x 513 axis 1 (271 MiB) +6%;
+// the hat table's column kernel (68-70 VGPRs, one workgroup per CU) +2 to +6%
+// on 257^3
+bool long_columns_preferred(int dtype, SweepParams const &p);
 // contiguous lines of period 2^L that go through the long-line kernel's LDS
 // staging buffer (a few lines per round)
 bool long_contiguous_staged(int dtype, int L);

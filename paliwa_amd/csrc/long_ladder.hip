@@ -2,7 +2,8 @@
 // float64 lines of 2^12 .. 2^15 points, float32 lines of 2^13 .. 2^16, for the
 // ladder schedules of the named transforms (hierarchize_in's and
 // dehierarchize_in's consecutive level lists with the reference's table
-// parities, paliwa/paliwa_transform.hpp:346-399).
+// parities, paliwa/paliwa_transform.hpp:346-399).  Also the column tiles of
+// strided float64 lines of 2^8 points (below).
 //
 // A workgroup of NT threads owns a tile of TB lines; each line of period n is
 // split into G = n / K chunks of K consecutive points, one chunk per thread in
@@ -24,7 +25,18 @@
 // (consecutive threads, consecutive points), and each thread takes its chunk
 // from a layout with one pad point after every chunk (chunks K points apart
 // would put a wavefront's reads in one bank); lines too long for the buffer
-// read their chunks straight from memory.  Each update is c0*g[lower] + c1*g[x] + c2*g[upper],
+// read their chunks straight from memory.
+//
+// Column tiles.  Strided float64 lines of 2^8 points (a 257^3 grid's first two
+// axes) run here too, with chunks of 16 points: a 1024-thread tile holds 64
+// columns, so each row access is 512 bytes (the fused kernel's 512-thread
+// tiles hold 32, and at odd pitches every 256-byte row spans three 128-byte
+// lines, the two at its ends shared with the neighbouring tiles), and 62
+// VGPRs with 24 KiB of LDS let two such workgroups share a CU, so one
+// computes while the other loads (the fused tile's LDS transpose buffer
+// leaves room for one at that width).
+//
+// Each update is c0*g[lower] + c1*g[x] + c2*g[upper],
 // left to right without contraction, exactly the reference's expression, and
 // a pass is a pure function of the state before it (written and read points
 // have opposite parity), so the results are bit-identical to the reference's.
@@ -44,6 +56,19 @@ constexpr int kLongThreads = 1024; // the widest tile; 256-thread tiles for few 
 constexpr int kStageBytes = 68 * 1024;
 
 template <typename T> constexpr int long_chunk() { return sizeof(T) == 8 ? 32 : 64; }
+// column tiles: strided float64 lines of 2^kColumnL points in chunks of
+// kColumnChunk
+constexpr int kColumnL = 8;
+constexpr int kColumnChunk = 16;
+// waves per SIMD a kernel is compiled for (amdgpu_waves_per_eu, through
+// __launch_bounds__'s second argument): 8 for 1024-thread tiles of chunks of
+// at most 128 bytes with two passes per level (the column tiles of the
+// biorthogonal and full-weighting tables: two workgroups per CU, 60 VGPRs),
+// else 1 (no bound; the one-pass hat table's column kernel takes 68-70 VGPRs,
+// and held to 64 it spills)
+template <typename T, int K, int NT, int PAT> constexpr int long_waves_per_eu() {
+  return NT == 1024 && K * int(sizeof(T)) <= 128 && PAT != kPatOdd ? 8 : 1;
+}
 
 // Padded lines of period 2^L that one staging round holds (0: the line does
 // not fit, its chunks are read from memory).
@@ -285,13 +310,14 @@ __device__ __forceinline__ void long_tiles(SweepParams const &p, Src const &src,
 }
 
 template <typename T, int K, int NT, bool STRIDED, int PAT, bool ASC>
-__global__ __launch_bounds__(NT) void long_ladder(SweepParams p, int64_t ntiles) {
+__global__ __launch_bounds__(NT, (long_waves_per_eu<T, K, NT, PAT>())) void long_ladder(SweepParams p,
+                                                                             int64_t ntiles) {
   long_tiles<T, K, NT, STRIDED, PAT, ASC>(p, OneGrid{p.data, p.A, p.B}, ntiles);
 }
 
 template <typename T, int K, int NT, bool STRIDED, int PAT, bool ASC>
-__global__ __launch_bounds__(NT) void long_ladder_batched(SweepParams p, int64_t ntiles,
-                                                          ManyGrids grids) {
+__global__ __launch_bounds__(NT, (long_waves_per_eu<T, K, NT, PAT>())) void long_ladder_batched(
+    SweepParams p, int64_t ntiles, ManyGrids grids) {
   long_tiles<T, K, NT, STRIDED, PAT, ASC>(p, grids, ntiles);
 }
 
@@ -307,10 +333,9 @@ hipError_t launch_long_kernel(Kernel kernel, SweepParams const &p, int64_t tiles
   return hipGetLastError();
 }
 
-template <typename T, bool STRIDED, bool BATCHED, int NT, typename... Extra>
+template <typename T, int K, bool STRIDED, bool BATCHED, int NT, typename... Extra>
 hipError_t dispatch_long_nt(SweepParams const &p, int64_t tiles, hipStream_t stream,
                             Extra... extra) {
-  constexpr int K = long_chunk<T>();
   bool const asc = p.schedule == kScheduleLadderAsc;
   auto go = [&](auto pat, auto asc_tag) -> hipError_t {
     constexpr int PAT = decltype(pat)::value;
@@ -340,23 +365,47 @@ hipError_t dispatch_long_nt(SweepParams const &p, int64_t tiles, hipStream_t str
 }
 
 // A line of G chunks fits a 256-thread tile when G <= 256.
-template <typename T, bool STRIDED, bool BATCHED, typename... Extra>
+template <typename T, int K, bool STRIDED, bool BATCHED, typename... Extra>
 hipError_t dispatch_long(SweepParams const &p, int threads, int64_t tiles, hipStream_t stream,
                          Extra... extra) {
   if (threads == 256)
-    return dispatch_long_nt<T, STRIDED, BATCHED, 256>(p, tiles, stream, extra...);
-  return dispatch_long_nt<T, STRIDED, BATCHED, kLongThreads>(p, tiles, stream, extra...);
+    return dispatch_long_nt<T, K, STRIDED, BATCHED, 256>(p, tiles, stream, extra...);
+  return dispatch_long_nt<T, K, STRIDED, BATCHED, kLongThreads>(p, tiles, stream, extra...);
+}
+
+// Tile width for `lines` lines of G chunks: 1024 threads, or 256 where
+// 1024-thread tiles would be fewer than the 256 CUs and a line fits 256
+// threads.
+int long_threads_of(int64_t G, int64_t lines) {
+  if (G > 256)
+    return kLongThreads;
+  int64_t const wide_tiles = (lines * G + kLongThreads - 1) / kLongThreads;
+  return wide_tiles < 256 ? 256 : kLongThreads;
+}
+
+bool long_columns_fit(int dtype, SweepParams const &p) {
+  return dtype == PALIWA_FLOAT64 && p.B > 1 && p.L == kColumnL;
 }
 
 template <typename T>
 hipError_t launch_long_typed(SweepParams const &p, hipStream_t stream) {
   int const dtype = sizeof(T) == 8 ? PALIWA_FLOAT64 : PALIWA_FLOAT32;
   int64_t const lines = p.A * p.B;
+  if constexpr (sizeof(T) == 8) {
+    if (long_columns_fit(dtype, p)) { // column tiles of 16-point chunks
+      int64_t const G = (int64_t(1) << p.L) / kColumnChunk;
+      int const threads = long_threads_of(G, lines);
+      int64_t const tb = threads / G;
+      return dispatch_long<T, kColumnChunk, true, false>(p, threads, (lines + tb - 1) / tb,
+                                                         stream);
+    }
+  }
   int const threads = long_threads_for(dtype, p.L, lines);
   int64_t const tb = long_tile_lines(dtype, p.L, threads);
   int64_t const tiles = (lines + tb - 1) / tb;
-  return p.B > 1 ? dispatch_long<T, true, false>(p, threads, tiles, stream)
-                 : dispatch_long<T, false, false>(p, threads, tiles, stream);
+  constexpr int K = long_chunk<T>();
+  return p.B > 1 ? dispatch_long<T, K, true, false>(p, threads, tiles, stream)
+                 : dispatch_long<T, K, false, false>(p, threads, tiles, stream);
 }
 
 } // namespace
@@ -365,7 +414,16 @@ bool long_ladder_fits(int dtype, SweepParams const &p) {
   int64_t const n = int64_t(1) << p.L;
   bool const ladder = (p.schedule == kScheduleLadderAsc || p.schedule == kScheduleLadderDesc) &&
                       p.pattern != kPatNone;
-  return ladder && n > fused_max_period(dtype) && n <= long_ladder_max_period(dtype);
+  return ladder && ((n > fused_max_period(dtype) && n <= long_ladder_max_period(dtype)) ||
+                    long_columns_fit(dtype, p));
+}
+
+bool long_columns_preferred(int dtype, SweepParams const &p) {
+  if (!long_ladder_fits(dtype, p) || !long_columns_fit(dtype, p) || p.pattern == kPatOdd)
+    return false;
+  int64_t const lines = p.A * p.B;
+  int64_t const bytes = lines * p.m * int64_t(sizeof(double));
+  return lines >= 16384 && (p.B >= 4096 || bytes <= (int64_t(160) << 20));
 }
 
 bool long_ladder_preferred(int dtype, SweepParams const &p) {
@@ -383,13 +441,7 @@ int long_ladder_max_period(int dtype) {
 }
 
 int long_threads_for(int dtype, int L, int64_t lines) {
-  int64_t const G = (int64_t(1) << L) / (dtype == PALIWA_FLOAT64 ? 32 : 64);
-  if (G > 256)
-    return kLongThreads;
-  // 1024-thread tiles of few lines leave CUs idle: 256-thread tiles (4 times
-  // as many workgroups) where the wide ones would be fewer than the 256 CUs
-  int64_t const wide_tiles = (lines * G + kLongThreads - 1) / kLongThreads;
-  return wide_tiles < 256 ? 256 : kLongThreads;
+  return long_threads_of((int64_t(1) << L) / (dtype == PALIWA_FLOAT64 ? 32 : 64), lines);
 }
 
 int64_t long_tile_lines(int dtype, int L, int threads) {
@@ -405,11 +457,12 @@ hipError_t launch_long_ladder(int dtype, SweepParams const &p, hipStream_t strea
 hipError_t launch_long_ladder_batched(int dtype, SweepParams const &p, bool strided,
                                       ManyGrids grids, int64_t tiles, hipStream_t stream) {
   int const threads = p.threads == 256 ? 256 : kLongThreads;
+  constexpr int K64 = long_chunk<double>(), K32 = long_chunk<float>();
   if (dtype == PALIWA_FLOAT64)
-    return strided ? dispatch_long<double, true, true>(p, threads, tiles, stream, grids)
-                   : dispatch_long<double, false, true>(p, threads, tiles, stream, grids);
-  return strided ? dispatch_long<float, true, true>(p, threads, tiles, stream, grids)
-                 : dispatch_long<float, false, true>(p, threads, tiles, stream, grids);
+    return strided ? dispatch_long<double, K64, true, true>(p, threads, tiles, stream, grids)
+                   : dispatch_long<double, K64, false, true>(p, threads, tiles, stream, grids);
+  return strided ? dispatch_long<float, K32, true, true>(p, threads, tiles, stream, grids)
+                 : dispatch_long<float, K32, false, true>(p, threads, tiles, stream, grids);
 }
 
 } // namespace paliwa_hip

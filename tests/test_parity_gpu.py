@@ -320,6 +320,37 @@ def test_long_ladder_contiguous_staged(pw, dtype, L):
                         f"{extents} {name} inv")
 
 
+@pytest.mark.parametrize("name", WAVELETS)
+def test_long_ladder_column_tiles(pw, name):
+    """Strided float64 lines of 256 points through the long-line kernel's
+    column tiles (16-point chunks): 1024-thread tiles of 64 columns over
+    16385 lines (the last tile holds one), over 16384 lines without images,
+    256-thread tiles over 25 lines and over an inner axis (1161 lines, 129
+    between consecutive points); the explicit path at two minimum levels in
+    both directions, and the automatic schedule over every axis (it takes the
+    column tiles for the two-pass tables on the 16384-line grids), against
+    the oracle."""
+    rng = np.random.default_rng(22)
+    cases = (([8, 14], [257, 16385], 0), ([8, 14], [256, 16384], 0),
+             ([8, 2, 2], [257, 5, 5], 0), ([3, 8, 7], [9, 257, 129], 1))
+    for level, extents, dim in cases:
+        a = closed_random(rng, extents, level, np.float64)
+        for lmin_L in (0, 3):
+            for fwd in (True, False):
+                levels = list(range(8, lmin_L, -1)) if fwd else list(range(lmin_L + 1, 9))
+                filters = (ob.FORWARD if fwd else ob.RECONSTRUCT)[name]
+                t = torch.from_numpy(a).cuda()
+                pw.transform_in(dim, t, level, level, levels, filters, path="long_ladder")
+                want = a.copy()
+                ob.transform_in(want, level, dim, levels, filters)
+                assert_same(t.cpu().numpy(), want,
+                            f"{extents} axis {dim} {name} lmin={lmin_L} fwd={fwd}")
+        lmin = [0] * len(level)
+        for fwd in (True, False):
+            assert_same(gpu_named(pw, a, level, lmin, name, fwd),
+                        oracle_named(a, level, lmin, name, fwd), f"{extents} {name} auto")
+
+
 def test_long_ladder_rejects_generic_lists(pw):
     """The long-line kernel runs ladders only: an explicit request with a
     non-consecutive level list is refused; the automatic schedule takes the
