@@ -64,10 +64,13 @@ extern "C" {
                                     returns PALIWA_ERR_UNSUPPORTED */
 
 /* Kernel schedules for paliwa_transform_in_path. */
-#define PALIWA_PATH_AUTO 0     /* fused where it holds the line, else the
-                                  long-line kernel (strided axes, contiguous
-                                  lines its staging buffer holds), else the
-                                  LDS-line kernel, else per pass */
+#define PALIWA_PATH_AUTO 0     /* the long-line kernel's column tiles for
+                                  strided float64 lines of 256 points on large
+                                  grids (two-pass tables), else fused where it
+                                  holds the line, else the long-line kernel
+                                  (strided axes, contiguous lines its staging
+                                  buffer holds), else the LDS-line kernel,
+                                  else per pass */
 #define PALIWA_PATH_FUSED 1    /* one launch per axis: lines staged in LDS,
                                   every level and pass in registers */
 #define PALIWA_PATH_LDS_LINE 2 /* one launch per axis: one line per workgroup
@@ -77,7 +80,9 @@ extern "C" {
                                   detail::lifting_pass */
 #define PALIWA_PATH_LONG_LADDER 4 /* lines longer than the fused kernel holds
                                      (float64 2^12..2^15, float32 2^13..2^16
-                                     points), named transforms' level lists:
+                                     points), and strided float64 lines of
+                                     2^8 points (column tiles of 16-point
+                                     chunks), named transforms' level lists:
                                      one launch per axis, a tile of lines
                                      across a workgroup, fine levels in
                                      registers, coarse ones in LDS */
