@@ -56,10 +56,12 @@ constexpr int kLongThreads = 1024; // the widest tile; 256-thread tiles for few 
 constexpr int kStageBytes = 68 * 1024;
 
 template <typename T> constexpr int long_chunk() { return sizeof(T) == 8 ? 32 : 64; }
-// column tiles: strided float64 lines of 2^kColumnL points in chunks of
-// kColumnChunk
+// column tiles: strided float64 lines of 2^kColumnL points in chunks of 128
+// bytes (16 points).  Float32 column tiles (32-point chunks, 128 columns per
+// tile) were bit-identical but gained at most 1-3% on the 257^3 float32 grid
+// (516 tiles, one per workgroup) and lost on smaller grids: not built.
 constexpr int kColumnL = 8;
-constexpr int kColumnChunk = 16;
+template <typename T> constexpr int column_chunk() { return 128 / int(sizeof(T)); }
 // waves per SIMD a kernel is compiled for (amdgpu_waves_per_eu, through
 // __launch_bounds__'s second argument): 8 for 1024-thread tiles of chunks of
 // at most 128 bytes with two passes per level (the column tiles of the
@@ -392,12 +394,12 @@ hipError_t launch_long_typed(SweepParams const &p, hipStream_t stream) {
   int const dtype = sizeof(T) == 8 ? PALIWA_FLOAT64 : PALIWA_FLOAT32;
   int64_t const lines = p.A * p.B;
   if constexpr (sizeof(T) == 8) {
-    if (long_columns_fit(dtype, p)) { // column tiles of 16-point chunks
-      int64_t const G = (int64_t(1) << p.L) / kColumnChunk;
+    if (long_columns_fit(dtype, p)) { // column tiles of 128-byte chunks
+      constexpr int K = column_chunk<T>();
+      int64_t const G = (int64_t(1) << p.L) / K;
       int const threads = long_threads_of(G, lines);
       int64_t const tb = threads / G;
-      return dispatch_long<T, kColumnChunk, true, false>(p, threads, (lines + tb - 1) / tb,
-                                                         stream);
+      return dispatch_long<T, K, true, false>(p, threads, (lines + tb - 1) / tb, stream);
     }
   }
   int const threads = long_threads_for(dtype, p.L, lines);
