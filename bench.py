@@ -68,7 +68,8 @@ def parse():
                     help="only run a 1 GiB device copy (counter calibration)")
     ap.add_argument("--pitch-probe", action="store_true",
                     help="only time the strided axis-0 sweep of 257 x X x Y float64 grids "
-                         "of about the 257^3 grid's size at several row pitches")
+                         "of about the 257^3 grid's size at several row pitches, with the "
+                         "fused kernel and with the column tiles")
     return ap.parse_args()
 
 
@@ -136,20 +137,24 @@ def cold_probe(torch, reps=5):
 def pitch_probe(torch, reps=50):
     """The strided sweep of axis 0 (lines of 257 points, biorthogonal,
     forward and inverse alternating, HIP events over `reps` of each) on
-    257 x X x Y float64 grids with X * Y within 6% of 257^2: the same kernel,
-    tile shape and bytes per line, rows X * Y * 8 bytes apart.  A row of a
-    32-column tile is 256 bytes; it starts on a 128-byte line boundary in
-    every row only when the pitch is a multiple of 16 points."""
+    257 x X x Y float64 grids with X * Y within 6% of 257^2, through two
+    explicit schedules: the fused kernel (32-column tiles, 256-byte rows) and
+    the long-line kernel's column tiles (64 columns, 512-byte rows).  Each
+    keeps its tile shape and bytes per line; only the distance between rows
+    (X * Y * 8 bytes) changes.  A tile row starts on a 128-byte line boundary
+    in every row only when the pitch is a multiple of 16 points."""
     import ctypes
     import math
 
-    from paliwa_amd import _lib
+    from paliwa_amd import _lib, transform
 
     stream = torch.cuda.current_stream()
 
     def level_of(x):
         return int(math.log2(x)) if x & (x - 1) == 0 else int(math.log2(x - 1))
 
+    filters = {True: transform.wavelet_filters("biorthogonal"),
+               False: transform.wavelet_filters("biorthogonal", reconstruct=True)}
     out = []
     shapes = ((256, 256), (2049, 32), (1025, 64), (513, 128), (512, 129), (257, 256),
               (257, 257), (129, 513), (65, 1025), (33, 2049))
@@ -158,28 +163,35 @@ def pitch_probe(torch, reps=50):
         shape = (257, X, Y)
         level = [8, level_of(X), level_of(Y)]
         g = torch.randn(shape, dtype=torch.float64, device="cuda")
-        args = (ctypes.c_void_p(g.data_ptr()), _lib.PALIWA_FLOAT64, 3, _lib.i64_array(shape),
-                _lib.i32_array(level), _lib.i32_array([0] * 3), _lib.i32_array(level), 0,
-                b"biorthogonal", ctypes.c_void_p(stream.cuda_stream))
-        fns = (_lib.lib.paliwa_hierarchize_in, _lib.lib.paliwa_dehierarchize_in)
-        for _ in range(3):
-            for fn in fns:
-                assert fn(*args) == 0, _lib.last_error()
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(reps):
-            for fn in fns:
-                fn(*args)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        us = e0.elapsed_time(e1) / (2 * reps) * 1e3
-        pitch = X * Y * 8
+        row = {"shape": list(shape), "pitch_bytes": X * Y * 8, "pitch_mod_128": X * Y * 8 % 128}
+        for path in ("fused", "long_ladder"):
+            calls = []
+            for fwd in (True, False):
+                levels = list(range(8, 0, -1)) if fwd else list(range(1, 9))
+                f = filters[fwd]
+                calls.append((ctypes.c_void_p(g.data_ptr()), _lib.PALIWA_FLOAT64, 3,
+                              _lib.i64_array(shape), _lib.i32_array(level), _lib.i32_array(level),
+                              0, _lib.i32_array(levels), len(levels), _lib.filter_array(f), len(f),
+                              _lib.PATHS[path], ctypes.c_void_p(stream.cuda_stream)))
+            fn = _lib.lib.paliwa_transform_in_path
+            for _ in range(3):
+                for a in calls:
+                    assert fn(*a) == 0, _lib.last_error()
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                for a in calls:
+                    fn(*a)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) / (2 * reps) * 1e3
+            name = "fused" if path == "fused" else "columns"
+            row[f"{name}_us"] = round(us, 2)
+            row[f"{name}_GBps"] = round(2 * g.numel() * 8 / (us * 1e-6) / 1e9, 1)
         if k >= len(shapes):
-            out.append({"shape": list(shape), "pitch_bytes": pitch,
-                        "pitch_mod_128": pitch % 128, "us": round(us, 2),
-                        "GBps": round(2 * g.numel() * 8 / (us * 1e-6) / 1e9, 1)})
+            out.append(row)
         del g
     print(json.dumps({"pitch_probe": out}))
 
