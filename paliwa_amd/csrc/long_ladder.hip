@@ -31,10 +31,10 @@
 // axes) run here too, with chunks of 16 points: a 1024-thread tile holds 64
 // columns, so each row access is 512 bytes (the fused kernel's 512-thread
 // tiles hold 32, and at odd pitches every 256-byte row spans three 128-byte
-// lines, the two at its ends shared with the neighbouring tiles), and 62
+// lines, the two at its ends shared with the neighbouring tiles), and 60
 // VGPRs with 24 KiB of LDS let two such workgroups share a CU, so one
-// computes while the other loads (the fused tile's LDS transpose buffer
-// leaves room for one at that width).
+// computes while the other loads (the fused kernel's LDS transpose buffer
+// for a tile that wide leaves room for one).
 //
 // Each update is c0*g[lower] + c1*g[x] + c2*g[upper],
 // left to right without contraction, exactly the reference's expression, and
