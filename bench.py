@@ -73,12 +73,13 @@ def parse():
     return ap.parse_args()
 
 
-def sweep_kernel(dim):
+def sweep_kernel(pw, extents, dim, forward):
     """The kernel the automatic schedule runs on axis `dim` of the 257^3
-    biorthogonal float64 grid: the long-line kernel's column tiles on the two
-    strided axes (lines of 256 points, 66049 of them, 136 MB), the fused
-    register kernel on the contiguous one."""
-    return "long_ladder column tiles" if dim < len(LEVEL) - 1 else "fused_ladder"
+    biorthogonal float64 grid, as the library reports it (paliwa_auto_path):
+    the long-line kernel's column tiles on the two strided axes (lines of 256
+    points), the fused register kernel on the contiguous one."""
+    name = pw.auto_path(dim, extents, LEVEL, [0] * len(LEVEL), WAVELET, forward)
+    return "long_ladder column tiles" if name == "long_ladder" and LEVEL[dim] == 8 else name
 
 
 def copy_probe(torch):
@@ -904,7 +905,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"{sweep_kernel(dominant[1])} {dominant[0]} dim{dominant[1]}",
+            "kernel": f"{sweep_kernel(pw, list(grid.shape), dominant[1], dominant[0] == 'fwd')} "
+                      f"{dominant[0]} dim{dominant[1]}",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

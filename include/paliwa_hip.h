@@ -134,6 +134,17 @@ int paliwa_transform_in_path(void *data, int32_t dtype, int32_t ndim,
                              const paliwa_filter *filters, int32_t nfilters,
                              int32_t path, void *stream);
 
+/* The kernel schedule (PALIWA_PATH_*) that PALIWA_PATH_AUTO takes for the
+ * named transform of `wavelet_name` along `dim` (forward: hierarchize_in's
+ * level list, else dehierarchize_in's), or PALIWA_PATH_AUTO itself when the
+ * axis has no level to sweep (minimum_level[dim] == level[dim]).  No
+ * counterpart in the reference: introspection of this library's schedule
+ * rules, for callers that tune around them.  Launches nothing, needs no
+ * device, and reads no grid data. */
+int paliwa_auto_path(int32_t dtype, int32_t ndim, const int64_t *extents,
+                     const int32_t *level, const int32_t *minimum_level, int32_t dim,
+                     const char *wavelet_name, int32_t forward, int32_t *path);
+
 /* Replaces paliwa::hierarchize_in<Dim> (paliwa/paliwa_transform.hpp:346-372):
  * levels level[dim], level[dim]-1, ..., minimum_level[dim]+1 with the forward
  * table of `wavelet_name` ("hat", "biorthogonal", "fullweighting"). */

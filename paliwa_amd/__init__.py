@@ -14,6 +14,7 @@ from .transform import (  # noqa: F401
     ShapeError,
     UnknownWaveletError,
     UnsupportedError,
+    auto_path,
     dehierarchize,
     dehierarchize_checked,
     dehierarchize_in,

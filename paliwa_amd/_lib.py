@@ -76,6 +76,10 @@ SIGNATURES = {
         ctypes.c_int,
         [_vp, _i32, _i32, _i64p, _i32p, _i32p, _i32, _i32p, _i32, _fp, _i32, _i32, _vp],
     ),
+    "paliwa_auto_path": (
+        ctypes.c_int,
+        [_i32, _i32, _i64p, _i32p, _i32p, _i32, ctypes.c_char_p, _i32, _i32p],
+    ),
     "paliwa_hierarchize_in": (
         ctypes.c_int,
         [_vp, _i32, _i32, _i64p, _i32p, _i32p, _i32p, _i32, ctypes.c_char_p, _vp],

@@ -164,4 +164,37 @@ int paliwa_dehierarchize(void *data, int32_t dtype, int32_t ndim,
                          stream);
 }
 
+int paliwa_auto_path(int32_t dtype, int32_t ndim, int64_t const *extents,
+                     int32_t const *level, int32_t const *minimum_level, int32_t dim,
+                     char const *wavelet_name, int32_t forward, int32_t *path) {
+  if (path == nullptr)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "path is NULL");
+  int64_t elements = 0;
+  int const unused = 0; // check_grid wants a data pointer; nothing is read
+  int rc = check_grid(&unused, dtype, ndim, extents, level, nullptr, &elements);
+  if (rc != PALIWA_OK)
+    return rc;
+  if (dim < 0 || dim >= ndim)
+    return fail(PALIWA_ERR_INVALID_ARGUMENT, "dim out of range");
+  if ((rc = check_level_range(ndim, level, minimum_level, level)) != PALIWA_OK)
+    return rc;
+  WaveletEntry const *w = find_wavelet(wavelet_name);
+  if (w == nullptr)
+    return fail(PALIWA_ERR_UNKNOWN_WAVELET, "map::at");
+  SweepParams p = make_params(nullptr, ndim, extents, level, dim);
+  if (forward)
+    set_filters(p, w->forward, w->nforward);
+  else
+    set_filters(p, w->reconstruct, w->nreconstruct);
+  set_named_levels(p, level[dim], minimum_level[dim], forward != 0);
+  if (p.nlevels == 0) {
+    *path = PALIWA_PATH_AUTO; // nothing to sweep
+  } else {
+    paliwa_hip::classify(p);
+    *path = auto_path(dtype, p);
+  }
+  paliwa_hip::last_error().clear();
+  return PALIWA_OK;
+}
+
 } // extern "C"

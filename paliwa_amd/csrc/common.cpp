@@ -137,24 +137,27 @@ SweepParams make_params(void *data, int32_t ndim, int64_t const *extents,
   return p;
 }
 
+int32_t auto_path(int32_t dtype, SweepParams const &p) {
+  int64_t const n = int64_t(1) << p.L;
+  if (paliwa_hip::long_columns_preferred(dtype, p))
+    return PALIWA_PATH_LONG_LADDER; // column tiles
+  if (n <= paliwa_hip::fused_max_period(dtype))
+    return PALIWA_PATH_FUSED;
+  if (paliwa_hip::long_ladder_preferred(dtype, p))
+    return PALIWA_PATH_LONG_LADDER;
+  if (n <= paliwa_hip::lds_line_max_period(dtype))
+    return PALIWA_PATH_LDS_LINE;
+  return PALIWA_PATH_PER_PASS;
+}
+
 // One axis sweep through the chosen kernel schedule.
 int sweep(int32_t dtype, SweepParams p, int32_t path, hipStream_t stream) {
   if (p.nlevels == 0 || p.npass == 0)
     return PALIWA_OK;
   paliwa_hip::classify(p);
   int64_t const n = int64_t(1) << p.L;
-  if (path == PALIWA_PATH_AUTO) {
-    if (paliwa_hip::long_columns_preferred(dtype, p))
-      path = PALIWA_PATH_LONG_LADDER; // column tiles
-    else if (n <= paliwa_hip::fused_max_period(dtype))
-      path = PALIWA_PATH_FUSED;
-    else if (paliwa_hip::long_ladder_preferred(dtype, p))
-      path = PALIWA_PATH_LONG_LADDER;
-    else if (n <= paliwa_hip::lds_line_max_period(dtype))
-      path = PALIWA_PATH_LDS_LINE;
-    else
-      path = PALIWA_PATH_PER_PASS;
-  }
+  if (path == PALIWA_PATH_AUTO)
+    path = auto_path(dtype, p);
   hipError_t err;
   switch (path) {
   case PALIWA_PATH_FUSED:

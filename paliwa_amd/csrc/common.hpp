@@ -47,6 +47,8 @@ int check_level_range(int32_t ndim, int32_t const *level,
 // The [A][m][B] view of axis `dim` of a dense row-major grid.
 SweepParams make_params(void *data, int32_t ndim, int64_t const *extents,
                         int32_t const *level, int32_t dim);
+// The schedule PALIWA_PATH_AUTO takes for a classified sweep (PALIWA_PATH_*).
+int32_t auto_path(int32_t dtype, SweepParams const &p);
 // One axis sweep through the chosen kernel schedule (PALIWA_PATH_*).
 int sweep(int32_t dtype, SweepParams p, int32_t path, hipStream_t stream);
 void set_filters(SweepParams &p, paliwa_filter const *filters,

@@ -230,6 +230,25 @@ def dehierarchize_in(dim: int, grid, level, minimum_level, maximum_level,
     return True
 
 
+def auto_path(dim: int, extents, level, minimum_level, wavelet_name: str = "hat",
+              forward: bool = True, dtype=torch.float64) -> str:
+    """The kernel schedule the automatic path takes for hierarchize_in
+    (forward) or dehierarchize_in along ``dim`` of a grid of these extents:
+    "fused", "long_ladder", "lds_line" or "per_pass", or "auto" when the axis
+    has no level to sweep.  Launches nothing and needs no device (an
+    introspection of this library's rules, with no counterpart in the
+    reference)."""
+    level = list(level)
+    code = {torch.float64: _lib.PALIWA_FLOAT64, torch.float32: _lib.PALIWA_FLOAT32}.get(dtype)
+    if code is None:
+        raise TypeError("dtype must be torch.float64 or torch.float32")
+    out = ctypes.c_int32(-1)
+    check(_lib.lib.paliwa_auto_path(code, len(level), _lib.i64_array(extents),
+                                    _lib.i32_array(level), _lib.i32_array(minimum_level), int(dim),
+                                    wavelet_name.encode(), int(bool(forward)), ctypes.byref(out)))
+    return {v: k for k, v in _lib.PATHS.items()}[out.value]
+
+
 def transform_in(dim: int, grid, level, maximum_level, levels: Iterable[int],
                  filters, stream=None, path: str = "auto") -> bool:
     """Any level list and filter list along one axis
