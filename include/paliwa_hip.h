@@ -66,11 +66,11 @@ extern "C" {
 /* Kernel schedules for paliwa_transform_in_path. */
 #define PALIWA_PATH_AUTO 0     /* the long-line kernel's column tiles for
                                   strided float64 lines of 256 points on large
-                                  grids (two-pass tables), else fused where it
-                                  holds the line, else the long-line kernel
-                                  (strided axes, contiguous lines its staging
-                                  buffer holds), else the LDS-line kernel,
-                                  else per pass */
+                                  grids (paliwa_auto_path reports the choice),
+                                  else fused where it holds the line, else the
+                                  long-line kernel (strided axes, contiguous
+                                  lines its staging buffer holds), else the
+                                  LDS-line kernel, else per pass */
 #define PALIWA_PATH_FUSED 1    /* one launch per axis: lines staged in LDS,
                                   every level and pass in registers */
 #define PALIWA_PATH_LDS_LINE 2 /* one launch per axis: one line per workgroup

@@ -200,13 +200,13 @@ bool long_ladder_fits(int dtype, SweepParams const &p);
 // than the LDS-line kernel holds
 bool long_ladder_preferred(int dtype, SweepParams const &p);
 // the automatic schedule's choice of the column tiles over the fused kernel:
-// filter tables of two passes per level, at least 16384 lines (1024-thread
-// tiles), and either at least 4096 lines between consecutive points (an outer
-// axis) or a grid of at most 160 MiB (Infinity-Cache-resident when swept
-// repeatedly); measured against the fused kernel (biorthogonal): 257^3 axes 0
-// and 1 -5 to -6%, 257 x 1025^2 axis 0 -4%, 257^2 x 513 axis 1 (271 MiB) +6%;
-// the hat table's column kernel (68-70 VGPRs, one workgroup per CU) +2 to +6%
-// on 257^3
+// at least 16384 lines (1024-thread tiles); for the two-pass tables, either
+// at least 4096 lines between consecutive points (an outer axis) or a grid of
+// at most 160 MiB (Infinity-Cache-resident when swept repeatedly); for the
+// hat table, an outer axis of a grid above 256 MiB.  Measured against the
+// fused kernel, biorthogonal: 257^3 axes 0 and 1 -4 to -6%, 257 x 1025^2
+// axis 0 -4%, 257^2 x 513 axis 1 (271 MiB) +6%; hat: 257 x 1025^2 axis 0
+// -9%, 257^3 axis 0 +1.5%, axis 1 -4%, 257^2 x 129 +5 to +6%
 bool long_columns_preferred(int dtype, SweepParams const &p);
 // contiguous lines of period 2^L that go through the long-line kernel's LDS
 // staging buffer (a few lines per round)

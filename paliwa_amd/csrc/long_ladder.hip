@@ -64,12 +64,13 @@ constexpr int kColumnL = 8;
 template <typename T> constexpr int column_chunk() { return 128 / int(sizeof(T)); }
 // waves per SIMD a kernel is compiled for (amdgpu_waves_per_eu, through
 // __launch_bounds__'s second argument): 8 for 1024-thread tiles of chunks of
-// at most 128 bytes with two passes per level (the column tiles of the
-// biorthogonal and full-weighting tables: two workgroups per CU, 60 VGPRs),
-// else 1 (no bound; the one-pass hat table's column kernel takes 68-70 VGPRs,
-// and held to 64 it spills)
-template <typename T, int K, int NT, int PAT> constexpr int long_waves_per_eu() {
-  return NT == 1024 && K * int(sizeof(T)) <= 128 && PAT != kPatOdd ? 8 : 1;
+// at most 128 bytes (the column tiles: two workgroups per CU; 60 VGPRs for
+// the two-pass tables, 58 for the hat table's dehierarchize), else 1 (no
+// bound).  The hat table's hierarchize (fine levels before coarse ones)
+// takes 72 VGPRs and held to 64 spills, so it runs unbounded, one workgroup
+// per CU.
+template <typename T, int K, int NT, int PAT, bool ASC> constexpr int long_waves_per_eu() {
+  return NT == 1024 && K * int(sizeof(T)) <= 128 && (PAT != kPatOdd || !ASC) ? 8 : 1;
 }
 
 // Padded lines of period 2^L that one staging round holds (0: the line does
@@ -232,10 +233,28 @@ __device__ __forceinline__ void long_tiles(SweepParams const &p, Src const &src,
     // the fine levels in the order of the tags (uniform conditions: every
     // thread of the workgroup takes the same branches to the barriers)
     auto fine_levels = [&](auto... e_tags) {
-      ((decltype(e_tags)::value >= p.lo && decltype(e_tags)::value <= p.hi
-            ? fine_level(e_tags)
-            : void()),
-       ...);
+      if constexpr (pat_passes<PAT>() == 1 && pat_odd<PAT>(0)) {
+        // odd passes only: every fine level needs the next chunk's point 0,
+        // which no odd pass at a stride below K changes, so one exchange
+        // serves them all
+        bool const any = ((decltype(e_tags)::value >= p.lo &&
+                           decltype(e_tags)::value <= p.hi) || ...);
+        if (any) {
+          xch[buf][tid] = v[0];
+          __syncthreads();
+          T const nb = xch[buf][geo.next];
+          buf ^= 1;
+          ((decltype(e_tags)::value >= p.lo && decltype(e_tags)::value <= p.hi
+                ? chunk_pass<T, K, (1 << decltype(e_tags)::value), true>(v, nb, cf.c[0])
+                : void()),
+           ...);
+        }
+      } else {
+        ((decltype(e_tags)::value >= p.lo && decltype(e_tags)::value <= p.hi
+              ? fine_level(e_tags)
+              : void()),
+         ...);
+      }
     };
     // coarse levels (stride 2^e >= K) on the chunks' first points, in LDS
     auto coarse_levels = [&]() {
@@ -312,13 +331,13 @@ __device__ __forceinline__ void long_tiles(SweepParams const &p, Src const &src,
 }
 
 template <typename T, int K, int NT, bool STRIDED, int PAT, bool ASC>
-__global__ __launch_bounds__(NT, (long_waves_per_eu<T, K, NT, PAT>())) void long_ladder(SweepParams p,
+__global__ __launch_bounds__(NT, (long_waves_per_eu<T, K, NT, PAT, ASC>())) void long_ladder(SweepParams p,
                                                                              int64_t ntiles) {
   long_tiles<T, K, NT, STRIDED, PAT, ASC>(p, OneGrid{p.data, p.A, p.B}, ntiles);
 }
 
 template <typename T, int K, int NT, bool STRIDED, int PAT, bool ASC>
-__global__ __launch_bounds__(NT, (long_waves_per_eu<T, K, NT, PAT>())) void long_ladder_batched(
+__global__ __launch_bounds__(NT, (long_waves_per_eu<T, K, NT, PAT, ASC>())) void long_ladder_batched(
     SweepParams p, int64_t ntiles, ManyGrids grids) {
   long_tiles<T, K, NT, STRIDED, PAT, ASC>(p, grids, ntiles);
 }
@@ -421,11 +440,15 @@ bool long_ladder_fits(int dtype, SweepParams const &p) {
 }
 
 bool long_columns_preferred(int dtype, SweepParams const &p) {
-  if (!long_ladder_fits(dtype, p) || !long_columns_fit(dtype, p) || p.pattern == kPatOdd)
+  if (!long_ladder_fits(dtype, p) || !long_columns_fit(dtype, p))
     return false;
   int64_t const lines = p.A * p.B;
   int64_t const bytes = lines * p.m * int64_t(sizeof(double));
-  return lines >= 16384 && (p.B >= 4096 || bytes <= (int64_t(160) << 20));
+  if (lines < 16384)
+    return false;
+  if (p.pattern == kPatOdd) // the hat table: only an outer axis streaming HBM
+    return p.B >= 4096 && bytes > (int64_t(256) << 20);
+  return p.B >= 4096 || bytes <= (int64_t(160) << 20);
 }
 
 bool long_ladder_preferred(int dtype, SweepParams const &p) {

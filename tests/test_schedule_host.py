@@ -16,14 +16,18 @@ def path(dim, extents, level, lmin=None, name="biorthogonal", forward=True, dtyp
 def test_column_tiles_on_the_257_cube(forward):
     """Strided float64 lines of 256 points with a two-pass table take the
     long-line kernel's column tiles; the contiguous axis, the one-pass hat
-    table and float32 stay on the fused kernel."""
+    table (except on an outer axis of a grid beyond the cache) and float32
+    stay on the fused kernel."""
     cube, lv = [257] * 3, [8] * 3
     for name in ("biorthogonal", "fullweighting"):
         assert path(0, cube, lv, name=name, forward=forward) == "long_ladder"
         assert path(1, cube, lv, name=name, forward=forward) == "long_ladder"
         assert path(2, cube, lv, name=name, forward=forward) == "fused"
     assert path(0, cube, lv, name="hat", forward=forward) == "fused"
+    assert path(1, cube, lv, name="hat", forward=forward) == "fused"
     assert path(0, cube, lv, dtype=torch.float32, forward=forward) == "fused"
+    # the hat table takes them only on an outer axis of a grid beyond the cache
+    assert path(0, [257, 1025, 1025], [8, 10, 10], name="hat", forward=forward) == "long_ladder"
 
 
 def test_column_tile_rule_bounds():
