@@ -351,6 +351,29 @@ def test_long_ladder_column_tiles(pw, name):
                         oracle_named(a, level, lmin, name, fwd), f"{extents} {name} auto")
 
 
+def test_auto_path_names_the_schedule_that_runs(pw):
+    """The schedule paliwa_auto_path reports for an axis, requested
+    explicitly, gives the automatic sweep's result bit for bit (every
+    schedule does; this checks the report names a schedule that holds the
+    line), on the 257^3 grid's axes for the three tables and on a grid whose
+    strided lines go to the column tiles under the hat table."""
+    rng = np.random.default_rng(23)
+    for extents, level in (([257, 257, 257], [8, 8, 8]), ([257, 129, 1025], [8, 7, 10])):
+        a = closed_random(rng, extents, level, np.float64)
+        for name in WAVELETS:
+            for dim in range(3):
+                for fwd in (True, False):
+                    path = pw.auto_path(dim, extents, level, [0, 0, 0], name, fwd)
+                    levels = list(range(level[dim], 0, -1)) if fwd else list(range(1, level[dim] + 1))
+                    filters = (ob.FORWARD if fwd else ob.RECONSTRUCT)[name]
+                    t = torch.from_numpy(a).cuda()
+                    (pw.hierarchize_in if fwd else pw.dehierarchize_in)(dim, t, level, [0, 0, 0],
+                                                                       level, name)
+                    u = torch.from_numpy(a).cuda()
+                    pw.transform_in(dim, u, level, level, levels, filters, path=path)
+                    assert torch.equal(t, u), (extents, name, dim, fwd, path)
+
+
 def test_long_ladder_rejects_generic_lists(pw):
     """The long-line kernel runs ladders only: an explicit request with a
     non-consecutive level list is refused; the automatic schedule takes the
